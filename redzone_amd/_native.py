@@ -1,0 +1,87 @@
+"""ctypes binding of librz_amd.so (the C ABI declared in include/redzone.h).
+
+The product path is the HIP library; if it is missing this module raises
+instead of falling back to anything else.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librz_amd.so")
+
+# status codes (rz_status)
+RZ_OK, RZ_EINVAL, RZ_ERANGE, RZ_EDUP, RZ_ECOOP, RZ_EHIP, RZ_ENOMEM, RZ_ENOTUNED = range(8)
+# flags
+RZ_PRESERVE_REMOVALS = 0x1
+RZ_VALIDATE = 0x2
+RZ_COOPERATIVE = 0x4
+RZ_STATS = 0x8
+RZ_PROFILE = 0x10
+
+
+class TileConfig(ctypes.Structure):
+    _fields_ = [
+        ("wg_size", ctypes.c_int32),
+        ("items", ctypes.c_int32),
+        ("max_slices", ctypes.c_int32),
+        ("variant", ctypes.c_int32),
+    ]
+
+    def __repr__(self):
+        return (f"TileConfig(wg_size={self.wg_size}, items={self.items}, "
+                f"max_slices={self.max_slices}, variant={self.variant})")
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int64),
+        ("k", ctypes.c_int64),
+        ("tail_removals", ctypes.c_int64),
+        ("orphan_pairs", ctypes.c_int64),
+        ("slices", ctypes.c_int64),
+        ("error_flags", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+    ]
+
+
+class RedzoneError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        msg = _lib.rz_status_string(status).decode() if _lib is not None else str(status)
+        super().__init__(f"{what}: {msg} (status {status})" if what else f"{msg} (status {status})")
+
+
+_lib = None
+
+# (name, restype, argtypes) of every exported function
+_vp, _i64, _u32, _i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint32, ctypes.c_int32
+SIGNATURES = [
+    ("rz_ctx_create", _i32, [ctypes.POINTER(_vp), _i32]),
+    ("rz_ctx_destroy", _i32, [_vp]),
+    ("rz_status_string", ctypes.c_char_p, [_i32]),
+    ("rz_device_info", _i32, [_vp, ctypes.POINTER(_i32), ctypes.c_char_p, _i32]),
+    ("rz_remove", _i32, [_vp, _vp, _i64, _u32, _vp, _i64, _u32, _vp]),
+    ("rz_remove_u32", _i32, [_vp, _vp, _i64, _vp, _i64, _u32, _vp]),
+    ("rz_remove_u64", _i32, [_vp, _vp, _i64, _vp, _i64, _u32, _vp]),
+    ("rz_remove_rec16", _i32, [_vp, _vp, _i64, _vp, _i64, _u32, _vp]),
+    ("rz_remove_config", _i32, [_vp, _vp, _i64, _u32, _vp, _i64, _u32, ctypes.POINTER(TileConfig), _vp]),
+    ("rz_last_stats", _i32, [_vp, ctypes.POINTER(Stats)]),
+    ("rz_last_profile", _i32, [_vp, ctypes.POINTER(ctypes.c_float)]),
+    ("rz_partition_on", _i32, [_vp, _vp, _i64, _u32, _vp, _i64, _u32, _vp]),
+]
+
+
+def load():
+    """Load librz_amd.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib

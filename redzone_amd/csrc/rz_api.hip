@@ -1,0 +1,232 @@
+// rz_api.hip -- C ABI entry points, context and scratch management.
+#include <string.h>
+
+#include "rz_ctx.hpp"
+#include "rz_device.hpp"
+
+namespace rz {
+
+__global__ __launch_bounds__(256) void rz_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint64_t count) {
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < count; i += gridDim.x * 256ull) p[i] = v;
+}
+
+rz_status fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s) {
+  if (count == 0) return RZ_OK;
+  const size_t g = (count + 255) / 256;
+  hipLaunchKernelGGL(rz_fill_u32, dim3((uint32_t)(g < 4096 ? g : 4096)), dim3(256), 0, s, p, v, (uint64_t)count);
+  RZ_HIP(hipGetLastError());
+  return RZ_OK;
+}
+
+// Grow a scratch buffer.  New memory is zeroed synchronously; the kernels keep
+// it zero.  Growing frees the old buffer (hipFree waits for the device), so do
+// it outside stream capture: rz_reserve exists for callers that capture.
+rz_status ensure(rz_ctx* ctx, DeviceBuffer& b, size_t bytes, hipStream_t s) {
+  (void)ctx;
+  if (bytes <= b.bytes) return RZ_OK;
+  size_t cap = b.bytes ? b.bytes : 4096;
+  while (cap < bytes) cap *= 2;
+  RZ_HIP(hipStreamSynchronize(s));
+  if (b.ptr) RZ_HIP(hipFree(b.ptr));
+  b.ptr = nullptr;
+  b.bytes = 0;
+  void* p = nullptr;
+  if (hipMalloc(&p, cap) != hipSuccess) return RZ_ENOMEM;
+  RZ_HIP(hipMemset(p, 0, cap));
+  b.ptr = p;
+  b.bytes = cap;
+  return RZ_OK;
+}
+
+rz_status prof_mark(rz_ctx* ctx, uint32_t flags, int i, hipStream_t s) {
+  if (!(flags & RZ_PROFILE)) return RZ_OK;
+  if (!ctx->ev[i]) RZ_HIP(hipEventCreate(&ctx->ev[i]));
+  RZ_HIP(hipEventRecord(ctx->ev[i], s));
+  if (i == 3) ctx->ev_recorded = true;
+  return RZ_OK;
+}
+
+rz_tile_config default_config(uint32_t esz, int64_t n, int64_t k) {
+  (void)esz;
+  (void)n;
+  rz_tile_config c;
+  c.wg_size = 256;
+  c.items = k >= (1 << 22) ? 8 : 4;
+  c.max_slices = 2048;
+  c.variant = 0;
+  return c;
+}
+
+static rz_status check_args(rz_ctx* ctx, const void* data, int64_t n, uint32_t esz, const int32_t* R, int64_t k) {
+  if (!ctx) return RZ_EINVAL;
+  if (n < 0 || k < 0 || k > n || n >= (1ll << 31)) return RZ_EINVAL;
+  if (esz_index(esz) < 0) return RZ_EINVAL;
+  if (k > 0 && (!data || !R)) return RZ_EINVAL;
+  if (data && ((uintptr_t)data % (esz > 8 ? 16 : esz)) != 0) return RZ_EINVAL;
+  if (R && ((uintptr_t)R % 4) != 0) return RZ_EINVAL;
+  return RZ_OK;
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
+    else prev = -1;
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+}  // namespace rz
+
+using namespace rz;
+
+extern "C" {
+
+rz_status rz_ctx_create(rz_ctx** out, int32_t device) {
+  if (!out) return RZ_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return RZ_EHIP;
+  DeviceGuard g(device);
+  hipDeviceProp_t prop;
+  RZ_HIP(hipGetDeviceProperties(&prop, device));
+  rz_ctx* ctx = new rz_ctx();
+  ctx->device = device;
+  ctx->cu_count = prop.multiProcessorCount;
+  strncpy(ctx->arch, prop.gcnArchName, sizeof(ctx->arch) - 1);
+  void* p = nullptr;
+  if (hipMalloc(&p, kCtrlWords * 4) != hipSuccess) { delete ctx; return RZ_ENOMEM; }
+  if (hipMemset(p, 0, kCtrlWords * 4) != hipSuccess) { (void)hipFree(p); delete ctx; return RZ_EHIP; }
+  ctx->ctrl.ptr = p;
+  ctx->ctrl.bytes = kCtrlWords * 4;
+  if (hipHostMalloc((void**)&ctx->ctrl_host, kCtrlWords * 4, hipHostMallocDefault) != hipSuccess) {
+    (void)hipFree(p);
+    delete ctx;
+    return RZ_ENOMEM;
+  }
+  memset(ctx->ctrl_host, 0, kCtrlWords * 4);
+  for (int e = 0; e < 3; ++e)
+    for (int b = 0; b < 33; ++b) ctx->table[e][b] = default_config(4u << e, 0, b ? (1ll << b) : 0);
+  *out = ctx;
+  return RZ_OK;
+}
+
+rz_status rz_ctx_destroy(rz_ctx* ctx) {
+  if (!ctx) return RZ_OK;
+  DeviceGuard g(ctx->device);
+  (void)hipDeviceSynchronize();
+  DeviceBuffer* bufs[] = {&ctx->tail_bits, &ctx->list_bits, &ctx->counts, &ctx->orphans, &ctx->status, &ctx->ctrl};
+  for (DeviceBuffer* b : bufs)
+    if (b->ptr) (void)hipFree(b->ptr);
+  if (ctx->ctrl_host) (void)hipHostFree(ctx->ctrl_host);
+  for (hipEvent_t e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete ctx;
+  return RZ_OK;
+}
+
+const char* rz_status_string(rz_status s) {
+  switch (s) {
+    case RZ_OK: return "ok";
+    case RZ_EINVAL: return "invalid argument";
+    case RZ_ERANGE: return "removal index out of range";
+    case RZ_EDUP: return "duplicate removal index";
+    case RZ_ECOOP: return "cooperative launch refused";
+    case RZ_EHIP: return "HIP runtime error";
+    case RZ_ENOMEM: return "out of device memory";
+    case RZ_ENOTUNED: return "no model loaded";
+  }
+  return "unknown status";
+}
+
+rz_status rz_device_info(rz_ctx* ctx, int32_t* cu_count, char* arch, int32_t arch_len) {
+  if (!ctx) return RZ_EINVAL;
+  if (cu_count) *cu_count = ctx->cu_count;
+  if (arch && arch_len > 0) {
+    strncpy(arch, ctx->arch, (size_t)arch_len - 1);
+    arch[arch_len - 1] = 0;
+  }
+  return RZ_OK;
+}
+
+rz_status rz_remove_config(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, int32_t* d_removals,
+                           int64_t k, uint32_t flags, const rz_tile_config* cfg, void* stream) {
+  rz_status st = check_args(ctx, d_data, n, elem_bytes, d_removals, k);
+  if (st != RZ_OK) return st;
+  if (!cfg) return RZ_EINVAL;
+  if (k == 0 || k == n) return RZ_OK;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  DeviceGuard g(ctx->device);
+  hipStream_t s = (hipStream_t)stream;
+  if (flags & RZ_VALIDATE) {
+    if ((st = launch_validate(ctx, d_removals, n, k, s)) != RZ_OK) return st;
+  }
+  st = launch_plain(ctx, d_data, n, elem_bytes, d_removals, k, flags, *cfg, s);
+  if (st == RZ_OK && (flags & RZ_STATS)) {
+    RZ_HIP(hipMemcpyAsync(ctx->ctrl_host, ctx->ctrl.ptr, kCtrlWords * 4, hipMemcpyDeviceToHost, s));
+    RZ_HIP(hipStreamSynchronize(s));
+  }
+  return st;
+}
+
+rz_status rz_remove(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, int32_t* d_removals, int64_t k,
+                    uint32_t flags, void* stream) {
+  rz_status st = check_args(ctx, d_data, n, elem_bytes, d_removals, k);
+  if (st != RZ_OK) return st;
+  if (k == 0 || k == n) return RZ_OK;
+  const rz_tile_config cfg = ctx->table[esz_index(elem_bytes)][log2_floor(k)];
+  return rz_remove_config(ctx, d_data, n, elem_bytes, d_removals, k, flags, &cfg, stream);
+}
+
+rz_status rz_remove_u32(rz_ctx* ctx, uint32_t* d_data, int64_t n, int32_t* d_removals, int64_t k, uint32_t flags,
+                        void* stream) {
+  return rz_remove(ctx, d_data, n, 4, d_removals, k, flags, stream);
+}
+rz_status rz_remove_u64(rz_ctx* ctx, uint64_t* d_data, int64_t n, int32_t* d_removals, int64_t k, uint32_t flags,
+                        void* stream) {
+  return rz_remove(ctx, d_data, n, 8, d_removals, k, flags, stream);
+}
+rz_status rz_remove_rec16(rz_ctx* ctx, void* d_data, int64_t n, int32_t* d_removals, int64_t k, uint32_t flags,
+                          void* stream) {
+  return rz_remove(ctx, d_data, n, 16, d_removals, k, flags, stream);
+}
+
+rz_status rz_last_stats(rz_ctx* ctx, rz_stats* out) {
+  if (!ctx || !out) return RZ_EINVAL;
+  memset(out, 0, sizeof(*out));
+  const uint32_t* h = ctx->ctrl_host;
+  out->tail_removals = (int64_t)(h[kCtrlStatTail] | ((uint64_t)h[kCtrlStatTail + 1] << 32));
+  out->orphan_pairs = (int64_t)(h[kCtrlStatOrphans] | ((uint64_t)h[kCtrlStatOrphans + 1] << 32));
+  out->error_flags = (int32_t)h[kCtrlError];
+  return RZ_OK;
+}
+
+rz_status rz_last_profile(rz_ctx* ctx, float ms[4]) {
+  if (!ctx || !ms) return RZ_EINVAL;
+  if (!ctx->ev_recorded) return RZ_EINVAL;
+  DeviceGuard g(ctx->device);
+  RZ_HIP(hipEventSynchronize(ctx->ev[3]));
+  RZ_HIP(hipEventElapsedTime(&ms[0], ctx->ev[0], ctx->ev[1]));
+  RZ_HIP(hipEventElapsedTime(&ms[1], ctx->ev[1], ctx->ev[2]));
+  RZ_HIP(hipEventElapsedTime(&ms[2], ctx->ev[2], ctx->ev[3]));
+  RZ_HIP(hipEventElapsedTime(&ms[3], ctx->ev[0], ctx->ev[3]));
+  return RZ_OK;
+}
+
+rz_status rz_partition_on(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, const int32_t* d_removals,
+                          int64_t k, uint32_t flags, void* stream) {
+  rz_status st = check_args(ctx, d_data, n, elem_bytes, d_removals, k);
+  if (st != RZ_OK) return st;
+  if (k == 0 || n == 0) return RZ_OK;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  DeviceGuard g(ctx->device);
+  hipStream_t s = (hipStream_t)stream;
+  if (flags & RZ_VALIDATE) {
+    if ((st = launch_validate(ctx, d_removals, n, k, s)) != RZ_OK) return st;
+  }
+  return launch_partition(ctx, d_data, n, elem_bytes, d_removals, k, flags, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+// rz_ctx.hpp -- host-side context: device scratch, tuning table, launch helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+
+#include "../../include/redzone.h"
+
+namespace rz {
+
+// Words of the control block (uint32 index).  Each group sits on its own
+// 128-byte line so that polling one does not contend with updates of another.
+enum CtrlWord : int {
+  kCtrlEpoch = 0,         // look-back epoch of the O(n) partition (bumped by its mark kernel)
+  kCtrlError = 32,        // device-detected contract violations (bit flags)
+  kCtrlBarCount = 64,     // uint64 arrival counter of the in-kernel grid barrier
+  kCtrlBarEpoch = 96,     // launch epoch of the in-kernel grid barrier
+  kCtrlStatTail = 128,    // uint64: tail removals k_z (RZ_STATS)
+  kCtrlStatOrphans = 160, // uint64: orphan pairs (RZ_STATS)
+  kCtrlValidate = 192,    // validation flags: 1 = out of range, 2 = duplicate
+  kCtrlWords = 256
+};
+
+enum ErrorBits : uint32_t {
+  kErrOrphanMismatch = 1u,  // dest and source orphan totals differ (duplicate or bad input)
+  kErrBarrierTimeout = 2u,  // an in-kernel wait gave up (workgroups not co-resident)
+  kErrLookbackTimeout = 4u, // a look-back wait gave up
+};
+
+constexpr int kMaxSlices = 4096;  // cap on move-kernel workgroups (LDS prefix in the pairing kernel)
+
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace rz
+
+struct rz_ctx {
+  int device = 0;
+  int cu_count = 0;
+  char arch[64] = {0};
+  int max_wg_per_cu_256 = 0;  // occupancy of the cooperative kernel at 256 threads
+  // scratch, all zero-initialised on allocation and kept zero by the kernels
+  rz::DeviceBuffer tail_bits;  // k bits: tail removals of the plain path
+  rz::DeviceBuffer list_bits;  // n bits: removals of the O(n) path and validation
+  rz::DeviceBuffer counts;     // per-slice orphan counts (uint64: dest | src << 32)
+  rz::DeviceBuffer orphans;    // dest orphans when RZ_PRESERVE_REMOVALS is set
+  rz::DeviceBuffer status;     // look-back status words of the partition
+  rz::DeviceBuffer ctrl;       // control block (rz::CtrlWord)
+  uint32_t* ctrl_host = nullptr;  // pinned mirror for stats / validation reads
+  rz_tile_config table[3][33];    // [element size 4/8/16][floor(log2 k)]
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // RZ_PROFILE marks
+  bool ev_recorded = false;
+  std::mutex mu;
+};
+
+namespace rz {
+
+rz_status ensure(rz_ctx* ctx, DeviceBuffer& b, size_t bytes, hipStream_t s);
+inline int esz_index(uint32_t esz) { return esz == 4 ? 0 : esz == 8 ? 1 : esz == 16 ? 2 : -1; }
+inline int log2_floor(int64_t v) {
+  int r = 0;
+  while ((v >> 1) > 0) { v >>= 1; ++r; }
+  return r;
+}
+rz_tile_config default_config(uint32_t esz, int64_t n, int64_t k);
+
+// launchers (one per translation unit)
+rz_status launch_plain(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, int32_t* R, int64_t k, uint32_t flags,
+                       const rz_tile_config& cfg, hipStream_t s);
+rz_status launch_partition(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, const int32_t* R, int64_t k,
+                           uint32_t flags, hipStream_t s);
+rz_status launch_validate(rz_ctx* ctx, const int32_t* R, int64_t n, int64_t k, hipStream_t s);
+// RZ_PROFILE: record mark i (0..3) on the stream when profiling
+rz_status prof_mark(rz_ctx* ctx, uint32_t flags, int i, hipStream_t s);
+// fill `count` uint32 words with `v` by a kernel (graph-capture safe, unlike hipMemsetAsync)
+rz_status fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s);
+
+}  // namespace rz
+
+#define RZ_HIP(call)                                  \
+  do {                                                \
+    hipError_t e_ = (call);                           \
+    if (e_ != hipSuccess) return RZ_EHIP;             \
+  } while (0)

@@ -1,0 +1,220 @@
+// rz_partition.hip -- O(n) stable in-place partition on gfx950, the comparator
+// and dispatch alternative of the Redzone path, plus input validation.
+//
+// The reference's O(n) paths mark each removal with a -1 sentinel
+// (remove_phase1, DataStruct.hxx:11-18) and compact items != -1 in one block
+// (StreamCompact_OneBlock, redzone.hxx:526-603) or with thrust::remove
+// (Thrust_remove.hxx:15-17).  Here:
+//   rz_mark_all   sets one bit per removal in an n-bit bitmap (payload
+//                 agnostic) and bumps the look-back epoch;
+//   rz_compact    one pass over all n items in tiles of WG*IPT, tile id =
+//                 workgroup id (workgroups are dispatched in id order).  Each
+//                 tile ranks its kept items with wave ballots and one LDS scan,
+//                 publishes its count, finds its output offset by decoupled
+//                 look-back over the predecessors' status words, and writes
+//                 its kept items in order.  A tile publishes only after all of
+//                 its loads have returned, and a tile writes only below the
+//                 end of its own input, so the in-place pass never overwrites
+//                 an item before it was read.  It clears the bitmap words it
+//                 read, so the bitmap is zero again for the next call.
+#include "rz_ctx.hpp"
+#include "rz_device.hpp"
+
+namespace rz {
+
+__global__ __launch_bounds__(256) void rz_mark_all(const int32_t* __restrict__ R, uint32_t k, uint32_t n,
+                                                    uint32_t* __restrict__ bits, uint32_t* __restrict__ ctrl) {
+  const uint32_t stride = gridDim.x * 256u;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < k; i += stride) {
+    const uint32_t r = (uint32_t)R[i];
+    if (r < n) or_relaxed(&bits[r >> 5], 1u << (r & 31));
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) add_relaxed(&ctrl[kCtrlEpoch], 1u);
+}
+
+// status word: [63:62] flag (0 none, 1 aggregate, 2 inclusive prefix), [61:32] epoch, [31:0] count
+constexpr uint64_t kFlagAgg = 1ull, kFlagIncl = 2ull;
+__device__ __forceinline__ uint64_t pack_status(uint64_t flag, uint32_t epoch, uint32_t v) {
+  return (flag << 62) | ((uint64_t)(epoch & 0x3fffffffu) << 32) | v;
+}
+
+template <typename T, int WG, int IPT>
+__global__ __launch_bounds__(WG) void rz_compact(T* __restrict__ A, uint32_t n, uint32_t* __restrict__ bits,
+                                                  uint64_t* __restrict__ status, uint32_t* __restrict__ ctrl) {
+  constexpr int NW = WG / kWave;
+  constexpr int E = IPT * NW;
+  constexpr int PER = (E + kWave - 1) / kWave;
+  __shared__ uint32_t s_off[E + 1];
+  __shared__ uint32_t s_excl;
+  const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+  const uint32_t t = blockIdx.x;
+  const uint32_t base = t * (uint32_t)(WG * IPT);
+  const uint32_t epoch = ld_relaxed(&ctrl[kCtrlEpoch]) & 0x3fffffffu;
+
+  T x[IPT];
+  uint32_t rank[IPT];
+  uint32_t keepbits = 0;
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t p = base + j * WG + tid;
+    uint32_t w = 0xffffffffu;
+    if (p < n) {
+      x[j] = A[p];
+      w = bits[p >> 5];
+    }
+    const bool keep = !((w >> (p & 31)) & 1u);
+    const uint64_t m = ballot(keep);
+    rank[j] = lane_rank(m);
+    keepbits |= (uint32_t)keep << j;
+    if (lane == 0) s_off[j * NW + wave] = (uint32_t)__popcll(m);
+  }
+  wait_all_memory();  // all item loads of this tile have returned
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t loc[PER];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int idx = lane * PER + q;
+      loc[q] = idx < E ? s_off[idx] : 0u;
+      sum += loc[q];
+    }
+    const uint32_t xs = wave_incl_scan(sum);
+    uint32_t ex = xs - sum;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int idx = lane * PER + q;
+      if (idx < E) s_off[idx] = ex;
+      ex += loc[q];
+    }
+    const uint32_t agg = __shfl(xs, kWave - 1, kWave);
+    // decoupled look-back
+    uint32_t excl = 0;
+    if (t == 0) {
+      if (lane == 0) st_relaxed(&status[0], pack_status(kFlagIncl, epoch, agg));
+    } else {
+      if (lane == 0) st_relaxed(&status[t], pack_status(kFlagAgg, epoch, agg));
+      int64_t j = (int64_t)t - 1;
+      uint32_t spins = 0;
+      while (true) {
+        const int64_t q = j - lane;
+        uint64_t st = q >= 0 ? ld_relaxed(&status[q]) : pack_status(kFlagIncl, epoch, 0u);
+        uint32_t flag = (((uint32_t)(st >> 32) & 0x3fffffffu) == epoch) ? (uint32_t)(st >> 62) : 0u;
+        if (q < 0) flag = (uint32_t)kFlagIncl;
+        const uint64_t incl_m = ballot(flag == kFlagIncl);
+        const uint64_t none_m = ballot(flag == 0u);
+        // lanes up to and including the nearest inclusive prefix must all be ready
+        const uint64_t upto = incl_m ? ((incl_m & (~incl_m + 1)) << 1) - 1 : ~0ull;
+        if (none_m & upto) {
+          if (++spins > (1u << 24)) {  // give up (the grid is not making progress)
+            if (lane == 0) or_relaxed(&ctrl[kCtrlError], kErrLookbackTimeout);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        const uint32_t v = ((upto >> lane) & 1ull) ? (uint32_t)st : 0u;
+        excl += wave_sum(v);
+        if (incl_m) break;
+        j -= kWave;
+      }
+      if (lane == 0) st_relaxed(&status[t], pack_status(kFlagIncl, epoch, excl + agg));
+    }
+    if (lane == 0) s_excl = excl;
+  }
+  __syncthreads();
+  const uint32_t excl = s_excl;
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    if ((keepbits >> j) & 1u) {
+      const uint32_t pos = excl + s_off[j * NW + wave] + rank[j];
+      if (pos < n) A[pos] = x[j];
+    }
+  }
+  // clear the bitmap words this tile read (tiles are word aligned)
+  const uint32_t w0 = base >> 5, w1 = min((base + WG * IPT + 31u) >> 5, (n + 31u) >> 5);
+  for (uint32_t w = w0 + tid; w < w1; w += WG) bits[w] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// validation: range and duplicates (uses the n-bit bitmap, cleared afterwards)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void rz_validate_mark(const int32_t* __restrict__ R, uint32_t k, uint32_t n,
+                                                         uint32_t* __restrict__ bits, uint32_t* __restrict__ flag) {
+  uint32_t bad = 0;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < k; i += gridDim.x * 256u) {
+    const uint32_t r = (uint32_t)R[i];
+    if (r >= n) { bad |= 1u; continue; }
+    const uint32_t bit = 1u << (r & 31);
+    if (or_relaxed(&bits[r >> 5], bit) & bit) bad |= 2u;
+  }
+  if (bad) or_relaxed(flag, bad);
+}
+
+__global__ __launch_bounds__(256) void rz_validate_clear(const int32_t* __restrict__ R, uint32_t k, uint32_t n,
+                                                          uint32_t* __restrict__ bits) {
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < k; i += gridDim.x * 256u) {
+    const uint32_t r = (uint32_t)R[i];
+    if (r < n) bits[r >> 5] = 0u;
+  }
+}
+
+rz_status launch_validate(rz_ctx* ctx, const int32_t* R, int64_t n, int64_t k, hipStream_t s) {
+  rz_status st;
+  if ((st = ensure(ctx, ctx->list_bits, (size_t)((n + 63) / 64) * 8, s)) != RZ_OK) return st;
+  uint32_t* ctrl = (uint32_t*)ctx->ctrl.ptr;
+  if ((st = fill_u32(ctrl + kCtrlValidate, 0u, 1, s)) != RZ_OK) return st;
+  const int64_t gw = (k + 255) / 256;
+  const dim3 g((uint32_t)(gw < (int64_t)ctx->cu_count * 8 ? (gw > 0 ? gw : 1) : ctx->cu_count * 8));
+  hipLaunchKernelGGL(rz_validate_mark, g, dim3(256), 0, s, R, (uint32_t)k, (uint32_t)n,
+                     (uint32_t*)ctx->list_bits.ptr, ctrl + kCtrlValidate);
+  hipLaunchKernelGGL(rz_validate_clear, g, dim3(256), 0, s, R, (uint32_t)k, (uint32_t)n,
+                     (uint32_t*)ctx->list_bits.ptr);
+  RZ_HIP(hipGetLastError());
+  RZ_HIP(hipMemcpyAsync(ctx->ctrl_host + kCtrlValidate, ctrl + kCtrlValidate, 4, hipMemcpyDeviceToHost, s));
+  RZ_HIP(hipStreamSynchronize(s));
+  const uint32_t f = ctx->ctrl_host[kCtrlValidate];
+  if (f & 1u) return RZ_ERANGE;
+  if (f & 2u) return RZ_EDUP;
+  return RZ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launch
+// ---------------------------------------------------------------------------
+constexpr int kPartWG = 256, kPartIPT = 16;
+
+template <typename T>
+static rz_status partition_impl(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, int64_t k, uint32_t flags,
+                                hipStream_t s) {
+  constexpr int TILE = kPartWG * kPartIPT;
+  const int64_t tiles = (n + TILE - 1) / TILE;
+  rz_status st;
+  if ((st = ensure(ctx, ctx->list_bits, (size_t)((n + 63) / 64) * 8, s)) != RZ_OK) return st;
+  if ((st = ensure(ctx, ctx->status, (size_t)tiles * 8, s)) != RZ_OK) return st;
+  uint32_t* bits = (uint32_t*)ctx->list_bits.ptr;
+  uint32_t* ctrl = (uint32_t*)ctx->ctrl.ptr;
+  const int64_t gw = (k + 255) / 256;
+  const dim3 g1((uint32_t)(gw < (int64_t)ctx->cu_count * 8 ? (gw > 0 ? gw : 1) : ctx->cu_count * 8));
+  if ((st = prof_mark(ctx, flags, 0, s)) != RZ_OK) return st;
+  hipLaunchKernelGGL(rz_mark_all, g1, dim3(256), 0, s, R, (uint32_t)k, (uint32_t)n, bits, ctrl);
+  RZ_HIP(hipGetLastError());
+  if ((st = prof_mark(ctx, flags, 1, s)) != RZ_OK) return st;
+  hipLaunchKernelGGL((rz_compact<T, kPartWG, kPartIPT>), dim3((uint32_t)tiles), dim3(kPartWG), 0, s, A, (uint32_t)n,
+                     bits, (uint64_t*)ctx->status.ptr, ctrl);
+  RZ_HIP(hipGetLastError());
+  if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;
+  return prof_mark(ctx, flags, 3, s);
+}
+
+rz_status launch_partition(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, const int32_t* R, int64_t k,
+                           uint32_t flags, hipStream_t s) {
+  switch (esz) {
+    case 4: return partition_impl<uint32_t>(ctx, (uint32_t*)data, n, R, k, flags, s);
+    case 8: return partition_impl<uint64_t>(ctx, (uint64_t*)data, n, R, k, flags, s);
+    case 16: return partition_impl<uint4>(ctx, (uint4*)data, n, R, k, flags, s);
+    default: return RZ_EINVAL;
+  }
+}
+
+}  // namespace rz

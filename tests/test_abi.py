@@ -1,0 +1,69 @@
+"""The C-ABI library loads and exports exactly what include/redzone.h declares
+(no compute calls: these run without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    names = set()
+    for fname in os.listdir(os.path.join(ROOT, "include")):
+        if not fname.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", fname)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"\b(rz_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    from redzone_amd import _native
+
+    lib = _native.load()
+    declared = _declared_functions()
+    assert "rz_remove" in declared and "rz_partition_on" in declared
+    for name in sorted(declared):
+        assert hasattr(lib, name), f"{name} declared in include/ but not exported"
+    bound = {n for n, _, _ in _native.SIGNATURES}
+    assert declared == bound, f"binding and header differ: {declared ^ bound}"
+
+
+def test_status_strings():
+    from redzone_amd import _native
+
+    lib = _native.load()
+    for s in range(8):
+        assert lib.rz_status_string(s)
+    assert lib.rz_status_string(0) == b"ok"
+
+
+def test_context_without_gpu_fails_cleanly():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from redzone_amd import _native
+
+    lib = _native.load()
+    h = ctypes.c_void_p()
+    assert lib.rz_ctx_create(ctypes.byref(h), 0) != 0
+    assert not h.value
+
+
+def test_null_context_rejected():
+    from redzone_amd import _native
+
+    lib = _native.load()
+    assert lib.rz_remove(None, None, 10, 4, None, 2, 0, None) == _native.RZ_EINVAL
+    assert lib.rz_partition_on(None, None, 10, 4, None, 2, 0, None) == _native.RZ_EINVAL
+
+
+def test_library_is_built_for_gfx950():
+    from redzone_amd import _native
+
+    blob = open(_native.LIB_PATH, "rb").read()
+    assert b"gfx950" in blob

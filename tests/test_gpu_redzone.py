@@ -1,0 +1,214 @@
+"""Parity of the HIP Redzone path (through the C ABI) with the CPU oracle.
+
+Bar: the survivors d[0, n-k) are array-identical to the canonical sequential
+oracle (which the oracle tests pin against the paper's worked examples and the
+Redzone_CPU restatement), and satisfy every invariant: length n-k, survivor
+multiset with payload bits, in-place invariant, no removed item left.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import redzone_amd as rz
+from redzone_amd import workloads as W
+from tests.parity import announce, expected_canonical, gpu_remove_parity
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "known_answers.json")
+
+
+def _payload(esz, n, seed):
+    return W.payload(esz, n, seed)
+
+
+def test_known_answers(ctx, oracle):
+    for case in json.load(open(GOLDEN))["cases"]:
+        announce("known_answer", case["name"])
+        n = case["n"]
+        if n == 0:
+            continue
+        A = torch.arange(n, dtype=torch.int32)
+        R = torch.tensor(case["R"], dtype=torch.int32)
+        got, _ = gpu_remove_parity(ctx, oracle, A, R)
+        assert got[: n - len(case["R"])].tolist() == case["expect"]
+
+
+def test_config1_index_payload(ctx, oracle):
+    """n=2^20 int32 A[i]=i, k=2^14 uniform distinct removals, unsorted; also
+    checked against the Redzone_CPU restatement."""
+    n, k = 1 << 20, 1 << 14
+    announce("config1", n, k)
+    A = W.index_payload(n)
+    R = W.uniform_removals(n, k, seed=1)
+    got, _ = gpu_remove_parity(ctx, oracle, A, R)
+    A2 = A.numpy().copy()
+    st, _ = oracle.redzone_cpu(A2, R.numpy().copy(), threads=7)
+    assert st == 0
+    assert np.array_equal(got[: n - k], A2[: n - k])
+
+
+@pytest.mark.parametrize("esz", [4, 8, 16])
+@pytest.mark.parametrize("pattern", ["uniform", "clustered", "tail_heavy"])
+@pytest.mark.parametrize("frac", [1e-3, 1e-2, 0.1, 0.5, 0.9])
+def test_patterns_and_sizes(ctx, oracle, esz, pattern, frac):
+    n = 1 << 18
+    k = int(n * frac)
+    announce("pattern", esz, pattern, frac)
+    A = _payload(esz, n, seed=11)
+    R = W.removals(pattern, n, k, seed=12)
+    gpu_remove_parity(ctx, oracle, A, R)
+
+
+CONFIGS = [rz.TileConfig(wg, it, ms, 0) for wg in (256, 512) for it in (2, 4, 8, 16) for ms in (1, 3, 64, 4096)]
+
+
+@pytest.mark.parametrize("esz", [4, 16])
+def test_launch_shape_invariance(ctx, oracle, esz):
+    """Every launch shape gives the identical array (canonical pairing is
+    independent of the slicing)."""
+    n = (1 << 17) + 123
+    k = int(n * 0.3)
+    A = _payload(esz, n, seed=21)
+    R = W.uniform_removals(n, k, seed=22)
+    want = expected_canonical(oracle, A.numpy(), R.numpy())
+    for cfg in CONFIGS:
+        announce("shape", esz, cfg)
+        gpu_remove_parity(ctx, oracle, A, R, config=cfg, want=want)
+
+
+@pytest.mark.parametrize("esz", [4, 8, 16])
+def test_cooperative_variant(ctx, oracle, esz):
+    """The single cooperative launch (in-kernel grid barriers) gives the same array."""
+    for n, frac, ms in [((1 << 18) + 5, 0.01, 4096), ((1 << 18) + 5, 0.4, 64), (1 << 16, 0.9, 7), (1 << 20, 0.05, 4096)]:
+        k = int(n * frac)
+        A = _payload(esz, n, seed=31)
+        R = W.tail_heavy_removals(n, k, seed=32)
+        for wg, it in ((256, 4), (512, 8)):
+            cfg = rz.TileConfig(wg, it, ms, 1)
+            announce("coop", esz, n, k, cfg)
+            gpu_remove_parity(ctx, oracle, A, R, config=cfg)
+
+
+def test_ragged_small(ctx, oracle):
+    rng = np.random.default_rng(41)
+    for trial in range(120):
+        n = int(rng.integers(1, 700))
+        k = int(rng.integers(0, n + 1))
+        esz = int(rng.choice([4, 8, 16]))
+        announce("ragged", trial, n, k, esz)
+        A = torch.from_numpy(rng.integers(-2 ** 31, 2 ** 31, size=(n, esz // 4), dtype=np.int64).astype(np.int32))
+        A = A.reshape(n) if esz == 4 else A
+        R = torch.from_numpy(rng.permutation(n)[:k].astype(np.int32))
+        cfg = rz.TileConfig(int(rng.choice([256, 512])), int(rng.choice([2, 4, 8, 16])), int(rng.integers(1, 50)), 0)
+        gpu_remove_parity(ctx, oracle, A, R, config=cfg if trial % 2 else None)
+
+
+@pytest.mark.parametrize("case", ["k1", "k_n_minus_1", "tail_only", "head_only", "k0", "kn"])
+def test_edges(ctx, oracle, case):
+    n = 100003
+    announce("edge", case)
+    A = W.unique_u32_payload(n, seed=51)
+    if case == "k1":
+        R = torch.tensor([777], dtype=torch.int32)
+    elif case == "k_n_minus_1":
+        R = W.uniform_removals(n, n - 1, seed=52)
+    elif case == "tail_only":
+        R = torch.flip(torch.arange(n - 5000, n, dtype=torch.int32), [0])
+    elif case == "head_only":
+        R = W.uniform_removals(n // 2, 5000, seed=53)
+    elif case == "k0":
+        R = torch.empty(0, dtype=torch.int32)
+    else:
+        R = W.uniform_removals(n, n, seed=54)
+    gpu_remove_parity(ctx, oracle, A, R)
+
+
+def test_preserve_removals(ctx, oracle):
+    n = 1 << 19
+    k = n // 20
+    announce("preserve", n, k)
+    A = W.unique_u32_payload(n, seed=61)
+    R = W.uniform_removals(n, k, seed=62)
+    _, dR = gpu_remove_parity(ctx, oracle, A, R, flags=rz.RZ_PRESERVE_REMOVALS)
+    assert torch.equal(dR.cpu(), R)
+
+
+def test_validate_rejects_bad_lists(ctx, oracle):
+    n = 5000
+    A = W.unique_u32_payload(n, seed=71)
+    dA = A.cuda()
+    announce("validate dup")
+    with pytest.raises(rz.RedzoneError) as e:
+        ctx.remove(dA, torch.tensor([5, 9, 5], dtype=torch.int32).cuda(), flags=rz.RZ_VALIDATE)
+    assert e.value.status == 3
+    announce("validate range")
+    with pytest.raises(rz.RedzoneError) as e:
+        ctx.remove(dA, torch.tensor([5, n], dtype=torch.int32).cuda(), flags=rz.RZ_VALIDATE)
+    assert e.value.status == 2
+    with pytest.raises(rz.RedzoneError):
+        ctx.remove(dA, torch.tensor([-1], dtype=torch.int32).cuda(), flags=rz.RZ_VALIDATE)
+    torch.cuda.synchronize()
+    assert torch.equal(dA.cpu(), A), "rejected calls must not touch the data"
+    announce("validate ok")
+    R = W.uniform_removals(n, 300, seed=72)
+    gpu_remove_parity(ctx, oracle, A, R, flags=rz.RZ_VALIDATE)
+
+
+def test_stats(ctx, oracle):
+    n = 1 << 18
+    k = n // 10
+    announce("stats", n, k)
+    A = W.index_payload(n)
+    R = W.tail_heavy_removals(n, k, seed=81)
+    gpu_remove_parity(ctx, oracle, A, R, flags=rz.RZ_STATS)
+    s = ctx.last_stats()
+    Rn = R.numpy()
+    sz = n - k
+    dead = np.zeros(k, bool)
+    dead[Rn[Rn >= sz] - sz] = True
+    dst_orphans = int(((Rn < sz) & dead).sum())
+    assert s.tail_removals == int((Rn >= sz).sum())
+    assert s.orphan_pairs == dst_orphans
+    assert s.error_flags == 0
+
+
+def test_full_size_u32_one_percent(ctx, oracle):
+    """n=2^28 unique uint32 payloads, k/n=1% uniform: array-exact against the oracle."""
+    n = 1 << 28
+    k = n // 100
+    announce("full 2^28 1%")
+    A = W.unique_u32_payload(n, seed=91, device="cuda")
+    R = W.uniform_removals(n, k, seed=92, device="cuda")
+    A_np = A.cpu().numpy()
+    R_np = R.cpu().numpy()
+    ctx.remove(A, R)
+    torch.cuda.synchronize()
+    got = A.cpu().numpy()
+    want = expected_canonical(oracle, A_np, R_np)
+    assert np.array_equal(got[: n - k], want[: n - k])
+    bad, fb = oracle.check(A_np, got, R_np)
+    assert bad == 0, (bad, fb)
+
+
+@pytest.mark.parametrize("order", ["sorted", "permuted"])
+def test_full_size_rec16(ctx, oracle, order):
+    """n=2^27 16-byte records, k=n/100 uniform, removal list sorted / permuted:
+    array-exact, float fields compared as bits."""
+    n = 1 << 27
+    k = n // 100
+    announce("full rec16", order)
+    A = W.rec16_payload(n, seed=101, device="cuda")
+    R = W.uniform_removals(n, k, seed=102, device="cuda")
+    if order == "sorted":
+        R = torch.sort(R).values
+    A_np = A.cpu().numpy()
+    R_np = R.cpu().numpy()
+    ctx.remove(A, R)
+    torch.cuda.synchronize()
+    got = A.cpu().numpy()
+    want = expected_canonical(oracle, A_np, R_np)
+    assert np.array_equal(got[: n - k], want[: n - k])
