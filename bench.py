@@ -1,0 +1,258 @@
+#!/usr/bin/env python3
+"""Benchmark of Redzone O(k) in-place removal on one MI355X.
+
+Workload (one step = one removal call on one list resident in HBM):
+  n = 2^28 uint32 unique payloads, k = floor(n/100) uniform distinct removals
+  (unsorted), seeded.  The call is destructive, so before every step the list
+  and the removal list are restored from pristine copies and the caches are
+  scrubbed by reading a 1 GiB buffer (cold-cache timing); the restore and the
+  scrub are outside the timed call.
+
+Printed: ONE JSON line on stdout.  `value` is list items per second of the
+removal call (n / t).  `roofline` is for the dominant kernel (rz_move), timed
+live with events on the launch stream; `cpu_baseline` is the oracle's
+restatement of Redzone_CPU timed on this host's cores (n/t, same workload).
+Everything else goes to stderr.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gitems/s & %HBM peak, n=2^28 k/n=1%; batched+learned-select vs plain"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--log2n", type=int, default=28)
+    p.add_argument("--frac", type=float, default=0.01)
+    p.add_argument("--pattern", default="uniform")
+    p.add_argument("--elem-bytes", type=int, default=4)
+    p.add_argument("--warm", action="store_true", help="skip the cache scrub before each call")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--no-extra", action="store_true", help="skip the comparator measurements")
+    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    return p.parse_args()
+
+
+def algorithmic_bytes(R, n, k, s, torch):
+    """B = 4k (read R) + s*k (read the tail) + s*(k - k_z) (write the moved items),
+    and the move kernel's share (orphan pairs are moved by the pairing kernel)."""
+    sz = n - k
+    tail = R >= sz
+    kz = int(tail.sum().item())
+    dead = torch.zeros(k, dtype=torch.bool, device=R.device)
+    dead[(R[tail] - sz).long()] = True
+    pairs = int(((~tail) & dead).sum().item())
+    B = 4 * k + s * k + s * (k - kz)
+    B_move = (4 + s) * k + s * (k - kz - pairs)
+    return B, B_move, kz, pairs
+
+
+class Scrubber:
+    def __init__(self, torch, nbytes=1 << 30):
+        self.buf = torch.ones(nbytes // 8, dtype=torch.int64, device="cuda")
+
+    def __call__(self):
+        return self.buf.sum()
+
+
+def time_calls(torch, ctx, fn_restore, fn_call, steps, warmup, scrub, profile):
+    """Per-call event times (ms) and per-kernel profiles of `steps` calls."""
+    times, profs = [], []
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    for i in range(warmup + steps):
+        fn_restore()
+        if scrub is not None:
+            scrub()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn_call()
+        e1.record()
+        if profile:
+            prof = ctx.last_profile()
+        e1.synchronize()
+        if i >= warmup:
+            times.append(e0.elapsed_time(e1))
+            if profile:
+                profs.append(prof)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    return times, profs, wall
+
+
+def cpu_baseline(A0_h, R0_h, n, k, seconds):
+    """The oracle's Redzone_CPU restatement on this host (O(k) restore between reps)."""
+    import numpy as np
+
+    from oracle import pyoracle
+
+    cores = max(1, min(16, os.cpu_count() or 1) - 1)  # the reference uses hardware_concurrency() - 1
+    A = A0_h.copy()
+    R = R0_h.copy()
+    sz = n - k
+    touched = np.concatenate([R0_h[R0_h < sz].astype(np.int64)])
+    ts = []
+    t_start = time.perf_counter()
+    while len(ts) < 20 and (time.perf_counter() - t_start) < seconds:
+        A[touched] = A0_h[touched]
+        A[sz:] = A0_h[sz:]
+        R[:] = R0_h
+        st, t = pyoracle.redzone_cpu(A, R, threads=cores)
+        assert st == 0
+        ts.append(t[0] / 1e9)
+    med = statistics.median(ts)
+    return {"value": round(n / med / 1e9, 3), "unit": "Gitems/s", "cores": cores, "kind": "port",
+            "sample": f"full workload (n=2^{n.bit_length() - 1}, k={k}), {len(ts)} reps, median "
+                      f"{med * 1e3:.2f} ms, Redzone_CPU restatement (OpenMP, {cores} threads)",
+            "ms_median": round(med * 1e3, 3)}, A
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(lists shard across GPUs without exchange; see the design notes). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import redzone_amd as rz
+    from redzone_amd import workloads as W
+
+    ctx = rz.Context(0)
+    info = ctx.device_info()
+    n = 1 << args.log2n
+    k = int(n * args.frac)
+    s = args.elem_bytes
+    log(f"device {info}, n=2^{args.log2n}, k={k}, elem {s} B, pattern {args.pattern}")
+    A0 = W.payload(s, n, seed=2026, device="cuda")
+    R0 = W.removals(args.pattern, n, k, seed=2027, device="cuda")
+    A = torch.empty_like(A0)
+    R = torch.empty_like(R0)
+    B, B_move, kz, pairs = algorithmic_bytes(R0, n, k, s, torch)
+    log(f"k_z={kz}, orphan pairs={pairs}, algorithmic bytes {B / 1e6:.2f} MB (move kernel {B_move / 1e6:.2f} MB)")
+    scrub = None if args.warm else Scrubber(torch)
+
+    def restore():
+        A.copy_(A0)
+        R.copy_(R0)
+
+    times, profs, wall = time_calls(torch, ctx, restore, lambda: ctx.remove(A, R, flags=rz.RZ_PROFILE),
+                                    args.steps, args.warmup, scrub, True)
+    t_med = statistics.median(times)
+    move_ms = statistics.mean(p[1] for p in profs)
+    mark_ms = statistics.mean(p[0] for p in profs)
+    pair_ms = statistics.mean(p[2] for p in profs)
+    log(f"redzone: median {t_med * 1e3:.1f} us (p10 {sorted(times)[len(times) // 10] * 1e3:.1f}, "
+        f"max {max(times) * 1e3:.1f}); kernels mark {mark_ms * 1e3:.1f} us, move {move_ms * 1e3:.1f} us, "
+        f"pair {pair_ms * 1e3:.1f} us")
+
+    # correctness of the last call against the oracle's canonical result (cheap: O(k) on the host)
+    import numpy as np
+
+    from oracle import pyoracle
+
+    A0_h = A0.cpu().numpy()
+    R0_h = R0.cpu().numpy()
+    want = A0_h.copy()
+    assert pyoracle.canonical(want, R0_h) == 0
+    got = A.cpu().numpy()
+    bit_exact = bool(np.array_equal(got[: n - k], want[: n - k]))
+    log(f"bit-exact against the oracle: {bit_exact}")
+
+    achieved = B_move / (move_ms * 1e-3) / 1e9
+    op_gbs = B / (t_med * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_rz_move_u32_n2e28_k1pct.json")
+    if os.path.exists(tpath) and args.log2n == 28 and args.frac == 0.01 and s == 4 and args.pattern == "uniform":
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    result = {
+        "metric": METRIC,
+        "value": round(n / (t_med * 1e-3) / 1e9, 2),
+        "unit": "Gitems/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_med, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {4: "u32", 8: "u64", 16: "rec16"}[s],
+        "data": "synthetic (seeded unique payloads, uniform distinct removals)",
+        "config": {"workload": f"redzone remove, n=2^{args.log2n} {s}-byte items, k/n={args.frac:g} {args.pattern}",
+                   "n": n, "k": k, "k_z": kz, "orphan_pairs": pairs, "elem_bytes": s,
+                   "cache": "warm" if args.warm else "cold (1 GiB read scrub before each call)",
+                   "path": "rz_remove (mark, move, pair kernels)"},
+        "roofline": {"bound": "hbm", "kernel": "rz_move", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": B_move, "avg_ms": round(move_ms, 5)},
+        "op": {"algorithmic_bytes": B, "GBps": round(op_gbs, 1), "hbm_frac": round(op_gbs / HBM_PEAK_GBS, 4),
+               "removals_per_s_G": round(k / (t_med * 1e-3) / 1e9, 3), "bit_exact": bit_exact,
+               "kernel_us": {"mark": round(mark_ms * 1e3, 2), "move": round(move_ms * 1e3, 2),
+                             "pair": round(pair_ms * 1e3, 2)},
+               "wall_ms_per_step_incl_restore": round(wall * 1e3 / (args.steps + args.warmup), 3)},
+    }
+
+    if not args.no_extra:
+        extra = {}
+        # the O(n) stable partition on the same list
+        t_on, _, _ = time_calls(torch, ctx, restore, lambda: ctx.partition(A, R), max(3, args.steps // 4), 1,
+                                scrub, False)
+        on_med = statistics.median(t_on)
+        extra["on_partition"] = {"ms": round(on_med, 4), "Gitems_s": round(n / (on_med * 1e-3) / 1e9, 2),
+                                 "redzone_speedup": round(on_med / t_med, 2),
+                                 "bytes": 4 * k + s * n + s * (n - k) + n // 8}
+        # k/n = 0.1%: Redzone vs the O(n) partition
+        k2 = n // 1000
+        R2 = W.removals(args.pattern, n, k2, seed=2028, device="cuda")
+        R2w = torch.empty_like(R2)
+
+        def restore2():
+            A.copy_(A0)
+            R2w.copy_(R2)
+
+        t_rz2, _, _ = time_calls(torch, ctx, restore2, lambda: ctx.remove(A, R2w), args.steps, 2, scrub, False)
+        t_on2, _, _ = time_calls(torch, ctx, restore2, lambda: ctx.partition(A, R2w), max(3, args.steps // 4), 1,
+                                 scrub, False)
+        rz2, on2 = statistics.median(t_rz2), statistics.median(t_on2)
+        extra["k0.1pct"] = {"k": k2, "redzone_ms": round(rz2, 4), "on_partition_ms": round(on2, 4),
+                            "redzone_vs_on": round(on2 / rz2, 1)}
+        # warm-cache figure of the headline call
+        t_warm, _, _ = time_calls(torch, ctx, restore, lambda: ctx.remove(A, R), args.steps, 2, None, False)
+        extra["warm_ms"] = round(statistics.median(t_warm), 4)
+        result["comparators"] = extra
+        log(f"comparators: {extra}")
+
+    if not args.no_cpu:
+        # restore the pristine list on the host copy for the CPU baseline
+        cb, A_cpu = cpu_baseline(A0_h, R0_h, n, k, args.cpu_seconds)
+        cb["matches_gpu"] = bool(np.array_equal(A_cpu[: n - k], want[: n - k]))
+        result["cpu_baseline"] = cb
+        log(f"cpu baseline: {cb}")
+    else:
+        result["cpu_baseline"] = None
+
+    print(json.dumps(result), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()
