@@ -45,7 +45,20 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-extra", action="store_true", help="skip the comparator measurements")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--variant", type=int, default=None, help="0 stream kernels, 1 cooperative, 2 persistent queue")
+    p.add_argument("--wg", type=int, default=256)
+    p.add_argument("--items", type=int, default=None)
+    p.add_argument("--max-slices", type=int, default=2048)
+    p.add_argument("--slice-steps", type=int, default=2)
     return p.parse_args()
+
+
+def tile_config(args, rz):
+    """Explicit launch shape when --variant is given; otherwise the context's tuned table."""
+    if args.variant is None:
+        return None
+    items = args.items or (8 if args.variant == 2 else 4)
+    return rz.TileConfig(args.wg, items, args.max_slices, args.variant, args.slice_steps, 1)
 
 
 def algorithmic_bytes(R, n, k, s, torch):
@@ -137,6 +150,7 @@ def main():
     from redzone_amd import workloads as W
 
     ctx = rz.Context(0)
+    cfg = tile_config(args, rz)
     info = ctx.device_info()
     n = 1 << args.log2n
     k = int(n * args.frac)
@@ -154,9 +168,13 @@ def main():
         A.copy_(A0)
         R.copy_(R0)
 
-    times, profs, wall = time_calls(torch, ctx, restore, lambda: ctx.remove(A, R, flags=rz.RZ_PROFILE),
-                                    args.steps, args.warmup, scrub, True)
+    # timed calls: no profiling events inside the call
+    times, _, wall = time_calls(torch, ctx, restore, lambda: ctx.remove(A, R, config=cfg), args.steps, args.warmup,
+                                scrub, False)
     t_med = statistics.median(times)
+    # per-kernel breakdown: the same calls with the library's events between its kernels
+    _, profs, _ = time_calls(torch, ctx, restore, lambda: ctx.remove(A, R, flags=rz.RZ_PROFILE, config=cfg),
+                             args.steps, 1, scrub, True)
     move_ms = statistics.mean(p[1] for p in profs)
     mark_ms = statistics.mean(p[0] for p in profs)
     pair_ms = statistics.mean(p[2] for p in profs)
@@ -229,14 +247,14 @@ def main():
             A.copy_(A0)
             R2w.copy_(R2)
 
-        t_rz2, _, _ = time_calls(torch, ctx, restore2, lambda: ctx.remove(A, R2w), args.steps, 2, scrub, False)
+        t_rz2, _, _ = time_calls(torch, ctx, restore2, lambda: ctx.remove(A, R2w, config=cfg), args.steps, 2, scrub, False)
         t_on2, _, _ = time_calls(torch, ctx, restore2, lambda: ctx.partition(A, R2w), max(3, args.steps // 4), 1,
                                  scrub, False)
         rz2, on2 = statistics.median(t_rz2), statistics.median(t_on2)
         extra["k0.1pct"] = {"k": k2, "redzone_ms": round(rz2, 4), "on_partition_ms": round(on2, 4),
                             "redzone_vs_on": round(on2 / rz2, 1)}
         # warm-cache figure of the headline call
-        t_warm, _, _ = time_calls(torch, ctx, restore, lambda: ctx.remove(A, R), args.steps, 2, None, False)
+        t_warm, _, _ = time_calls(torch, ctx, restore, lambda: ctx.remove(A, R, config=cfg), args.steps, 2, None, False)
         extra["warm_ms"] = round(statistics.median(t_warm), 4)
         result["comparators"] = extra
         log(f"comparators: {extra}")

@@ -54,13 +54,15 @@ typedef enum rz_status {
 
 typedef struct rz_ctx rz_ctx;
 
-/* Launch shape of the plain path: workgroup size, items per thread and the
- * cap on the number of removal slices (one slice per workgroup). */
+/* Launch shape.  A step is wg_size * items pairs; the removal list is cut into
+ * slices of whole steps, one workgroup (or queue item) per slice. */
 typedef struct rz_tile_config {
   int32_t wg_size;     /* 256 or 512 */
-  int32_t items;       /* items per thread per step: 2, 4, 8 or 16 */
-  int32_t max_slices;  /* upper bound on workgroups of the move kernel */
-  int32_t variant;     /* 0 = three kernels on the stream, 1 = one cooperative kernel */
+  int32_t items;       /* items per thread per step: 2, 4, 8, 16 (variants 0/1); 4 or 8 (variant 2, batches) */
+  int32_t max_slices;  /* variants 0/1: upper bound on slices (workgroups of the move step) */
+  int32_t variant;     /* 0 = three kernels on the stream, 1 = one cooperative kernel, 2 = persistent queue */
+  int32_t slice_steps; /* variant 2 and batches: steps per slice (0 = 4) */
+  int32_t chunk;       /* variant 2 and batches: queue items taken per ticket (0 = 1) */
 } rz_tile_config;
 
 /* Statistics of the last call made with RZ_STATS. */
@@ -117,6 +119,86 @@ rz_status rz_last_profile(rz_ctx* ctx, float ms[4]);
  * is not modified. */
 rz_status rz_partition_on(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, const int32_t* d_removals,
                           int64_t k, uint32_t flags, void* stream);
+
+/* ---- batched Redzone ----------------------------------------------------- */
+/* No counterpart in the reference (it removes from one list per launch).
+ * Many independent lists are compacted by one persistent kernel that pulls
+ * (list, slice) work items from a device queue; every list gets exactly the
+ * result rz_remove would give it.  Lists must not overlap. */
+typedef struct rz_list_desc {
+  void* data;          /* device: n items of elem_bytes each */
+  int32_t* removals;   /* device: k distinct indices in [0, n) */
+  int64_t n;
+  int64_t k;
+  uint32_t elem_bytes; /* 4, 8 or 16 */
+  uint32_t method;     /* 0 = Redzone (the only method the queue runs) */
+} rz_list_desc;
+
+typedef struct rz_batch rz_batch;
+
+typedef struct rz_batch_info {
+  int64_t nitems;        /* work items in the queue */
+  int64_t bitmap_bytes;  /* tail bitmap scratch */
+  int32_t grid;          /* persistent workgroups launched */
+  int32_t lists_active;  /* lists with 0 < k < n */
+} rz_batch_info;
+
+/* Plans a batch from a HOST array of descriptors (pointers are device
+ * pointers) and allocates its scratch; uploads the plan synchronously.  The
+ * batch can then be launched any number of times on the same buffers (new
+ * data, same shapes).  cfg may be null (context default). */
+rz_status rz_batch_create(rz_ctx* ctx, const rz_list_desc* lists, int32_t nlists, uint32_t flags,
+                          const rz_tile_config* cfg, rz_batch** out);
+/* One persistent launch (or a replay of the captured graph); no allocation,
+ * no memset, so it may itself be captured by the caller. */
+rz_status rz_batch_launch(rz_batch* b, void* stream);
+/* Captures the launch into a hipGraph owned by the batch; later
+ * rz_batch_launch calls replay it. */
+rz_status rz_batch_capture(rz_batch* b, void* stream);
+rz_status rz_batch_get_info(rz_batch* b, rz_batch_info* out);
+/* Waits for the device, then frees the batch. */
+rz_status rz_batch_destroy(rz_batch* b);
+/* One-shot: plan + launch.  The plan is kept in the context until the next
+ * call (or rz_ctx_destroy), so the launch stays asynchronous. */
+rz_status rz_remove_batched(rz_ctx* ctx, const rz_list_desc* lists, int32_t nlists, uint32_t flags, void* stream);
+
+/* ---- synthetic inputs ---------------------------------------------------- */
+/* Device generators of benchmark and tuning inputs.  No counterpart in the
+ * reference, whose generator (TestLists_t::Update, DataStruct.hxx:73-133)
+ * draws on the host.  rz_gen_uniform_removals writes k distinct indices in
+ * [0, n): the first k images of a keyed bijection of [0, n), in that order. */
+rz_status rz_gen_uniform_removals(rz_ctx* ctx, int32_t* d_removals, int64_t n, int64_t k, uint64_t seed,
+                                  void* stream);
+/* n items of elem_bytes: the first 4-byte word of item i is a keyed bijection
+ * of i (unique), the other words are hashes of i. */
+rz_status rz_gen_payload(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, uint64_t seed, void* stream);
+
+/* ---- autotuner ----------------------------------------------------------- */
+/* No counterpart in the reference (its launch shape is fixed at compile time:
+ * GPUBlockCount / GPUBlockSize, tests.cu:1-2).  The context keeps one launch
+ * shape per (element size, floor(log2 k)) for rz_remove and one for batches. */
+/* Built-in candidate lists (batch = 0: plain path, 1: batches). */
+rz_status rz_tune_candidates(int32_t batch, rz_tile_config* out, int32_t max, int32_t* count);
+/* Times every candidate (null = built-in list) on a synthetic list of n items
+ * with k uniform removals (median of reps calls; cold = read a 1 GiB scrub
+ * buffer before each call) and installs the fastest for (elem_bytes, log2 k).
+ * Allocates and frees its own inputs; synchronises. */
+rz_status rz_tune(rz_ctx* ctx, uint32_t elem_bytes, int64_t n, int64_t k, const rz_tile_config* candidates,
+                  int32_t ncand, int32_t reps, int32_t cold, rz_tile_config* best, float* best_ms, float* all_ms);
+/* The same for batches: nlists lists, n log-uniform in [n_min, n_max],
+ * k = floor(r n) with r uniform in [0, r_max]. */
+rz_status rz_tune_batch(rz_ctx* ctx, uint32_t elem_bytes, int32_t nlists, int64_t n_min, int64_t n_max,
+                        double r_max, int32_t reps, const rz_tile_config* candidates, int32_t ncand,
+                        rz_tile_config* best, float* best_ms, float* all_ms);
+rz_status rz_table_get(rz_ctx* ctx, uint32_t elem_bytes, int32_t log2k, rz_tile_config* out);
+rz_status rz_table_set(rz_ctx* ctx, uint32_t elem_bytes, int32_t log2k, const rz_tile_config* cfg);
+rz_status rz_batch_config_get(rz_ctx* ctx, rz_tile_config* out);
+rz_status rz_batch_config_set(rz_ctx* ctx, const rz_tile_config* cfg);
+/* Saves / loads the whole table as text ("plain esz log2k wg items max_slices
+ * variant slice_steps chunk" and "batch wg items max_slices variant
+ * slice_steps chunk" lines). */
+rz_status rz_table_save(rz_ctx* ctx, const char* path);
+rz_status rz_table_load(rz_ctx* ctx, const char* path);
 
 #ifdef __cplusplus
 }

@@ -11,12 +11,13 @@ plain device memory; every computation happens in the library's HIP kernels.
     new_len = ctx.partition(data, removals)       # O(n) stable partition
 """
 import ctypes
+import os
 
 from . import _native
-from ._native import (RZ_COOPERATIVE, RZ_PRESERVE_REMOVALS, RZ_PROFILE, RZ_STATS, RZ_VALIDATE, RedzoneError,
-                      Stats, TileConfig)
+from ._native import (RZ_COOPERATIVE, RZ_PRESERVE_REMOVALS, RZ_PROFILE, RZ_STATS, RZ_VALIDATE, BatchInfo, ListDesc,
+                      RedzoneError, Stats, TileConfig)
 
-__all__ = ["Context", "remove", "partition", "TileConfig", "Stats", "RedzoneError", "RZ_PRESERVE_REMOVALS",
+__all__ = ["Context", "Batch", "remove", "partition", "TileConfig", "Stats", "RedzoneError", "RZ_PRESERVE_REMOVALS",
            "RZ_VALIDATE", "RZ_COOPERATIVE", "RZ_STATS", "RZ_PROFILE", "elem_bytes_of", "default_context"]
 
 
@@ -48,7 +49,7 @@ def _check_tensor(t, name, device):
 class Context:
     """Owns an rz_ctx (device scratch + tuning table) on one device."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, load_tuned=True):
         self._lib = _native.load()
         self.device = int(device)
         h = ctypes.c_void_p()
@@ -56,6 +57,10 @@ class Context:
         if st != 0:
             raise RedzoneError(st, "rz_ctx_create")
         self._h = h
+        self.tuned_table = None
+        if load_tuned and os.path.exists(_native.TUNED_TABLE) and self.device_info()["arch"].startswith("gfx950"):
+            self.load_table(_native.TUNED_TABLE)
+            self.tuned_table = _native.TUNED_TABLE
 
     def close(self):
         if getattr(self, "_h", None):
@@ -130,6 +135,126 @@ class Context:
         out = Stats()
         self._call("rz_last_stats", self._h, ctypes.byref(out))
         return out
+
+    # ---- synthetic inputs --------------------------------------------------
+    def gen_uniform_removals(self, removals, n, seed, stream=None):
+        """Fills the int32 device tensor `removals` (k entries) with k distinct indices of [0, n)."""
+        _check_tensor(removals, "removals", self.device)
+        self._call("rz_gen_uniform_removals", self._h, ctypes.c_void_p(removals.data_ptr()), int(n),
+                   removals.shape[0], int(seed), _stream_handle(stream, self.device))
+
+    def gen_payload(self, data, seed, stream=None):
+        _check_tensor(data, "data", self.device)
+        self._call("rz_gen_payload", self._h, ctypes.c_void_p(data.data_ptr()), data.shape[0], elem_bytes_of(data),
+                   int(seed), _stream_handle(stream, self.device))
+
+    # ---- autotuner ---------------------------------------------------------
+    @staticmethod
+    def candidates(batch=False):
+        lib = _native.load()
+        cnt = ctypes.c_int32()
+        lib.rz_tune_candidates(1 if batch else 0, None, 0, ctypes.byref(cnt))
+        arr = (TileConfig * cnt.value)()
+        lib.rz_tune_candidates(1 if batch else 0, arr, cnt.value, ctypes.byref(cnt))
+        return list(arr)
+
+    def tune(self, elem_bytes, n, k, candidates=None, reps=5, cold=True):
+        """Times every candidate on a synthetic list, installs the fastest; returns
+        (best, best_ms, [(cfg, ms), ...]) with ms = -1 for shapes the device refused."""
+        cands = candidates or self.candidates(False)
+        arr = (TileConfig * len(cands))(*cands)
+        all_ms = (ctypes.c_float * len(cands))()
+        best = TileConfig()
+        best_ms = ctypes.c_float()
+        self._call("rz_tune", self._h, elem_bytes, int(n), int(k), arr, len(cands), reps, 1 if cold else 0,
+                   ctypes.byref(best), ctypes.byref(best_ms), all_ms)
+        return best, best_ms.value, list(zip(cands, list(all_ms)))
+
+    def tune_batch(self, elem_bytes, nlists, n_min, n_max, r_max, reps=3, candidates=None):
+        cands = candidates or self.candidates(True)
+        arr = (TileConfig * len(cands))(*cands)
+        all_ms = (ctypes.c_float * len(cands))()
+        best = TileConfig()
+        best_ms = ctypes.c_float()
+        self._call("rz_tune_batch", self._h, elem_bytes, nlists, int(n_min), int(n_max), float(r_max), reps, arr,
+                   len(cands), ctypes.byref(best), ctypes.byref(best_ms), all_ms)
+        return best, best_ms.value, list(zip(cands, list(all_ms)))
+
+    def table(self, elem_bytes, log2k):
+        out = TileConfig()
+        self._call("rz_table_get", self._h, elem_bytes, log2k, ctypes.byref(out))
+        return out
+
+    def set_table(self, elem_bytes, log2k, cfg):
+        self._call("rz_table_set", self._h, elem_bytes, log2k, ctypes.byref(cfg))
+
+    def batch_config(self):
+        out = TileConfig()
+        self._call("rz_batch_config_get", self._h, ctypes.byref(out))
+        return out
+
+    def set_batch_config(self, cfg):
+        self._call("rz_batch_config_set", self._h, ctypes.byref(cfg))
+
+    def save_table(self, path):
+        self._call("rz_table_save", self._h, os.fsencode(path))
+
+    def load_table(self, path):
+        self._call("rz_table_load", self._h, os.fsencode(path))
+
+    def _descs(self, lists):
+        """lists: iterable of (data, removals) device tensors -> ctypes array of ListDesc."""
+        lists = list(lists)
+        arr = (ListDesc * max(1, len(lists)))()
+        for i, (data, rem) in enumerate(lists):
+            n, k, esz = self._args(data, rem, None, None)
+            arr[i] = ListDesc(ctypes.c_void_p(data.data_ptr()), ctypes.c_void_p(rem.data_ptr()), n, k, esz, 0)
+        return arr, len(lists)
+
+    def batch(self, lists, flags=0, config=None):
+        """Plans a batch over [(data, removals), ...]; returns a Batch."""
+        arr, cnt = self._descs(lists)
+        h = ctypes.c_void_p()
+        cfg = ctypes.byref(config) if config is not None else None
+        self._call("rz_batch_create", self._h, arr, cnt, flags, cfg, ctypes.byref(h))
+        return Batch(self, h)
+
+    def remove_batched(self, lists, flags=0, stream=None):
+        """One-shot batched removal over [(data, removals), ...]."""
+        arr, cnt = self._descs(lists)
+        self._call("rz_remove_batched", self._h, arr, cnt, flags, _stream_handle(stream, self.device))
+
+
+class Batch:
+    """A planned batch of lists (rz_batch): launch it, or capture it in a hipGraph and replay."""
+
+    def __init__(self, ctx, handle):
+        self._ctx = ctx
+        self._lib = ctx._lib
+        self._h = handle
+
+    def launch(self, stream=None):
+        self._ctx._call("rz_batch_launch", self._h, _stream_handle(stream, self._ctx.device))
+
+    def capture(self, stream=None):
+        self._ctx._call("rz_batch_capture", self._h, _stream_handle(stream, self._ctx.device))
+
+    def info(self):
+        out = BatchInfo()
+        self._ctx._call("rz_batch_get_info", self._h, ctypes.byref(out))
+        return {"nitems": out.nitems, "bitmap_bytes": out.bitmap_bytes, "grid": out.grid,
+                "lists_active": out.lists_active}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.rz_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _default = {}

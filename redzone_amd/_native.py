@@ -25,11 +25,36 @@ class TileConfig(ctypes.Structure):
         ("items", ctypes.c_int32),
         ("max_slices", ctypes.c_int32),
         ("variant", ctypes.c_int32),
+        ("slice_steps", ctypes.c_int32),
+        ("chunk", ctypes.c_int32),
     ]
 
     def __repr__(self):
-        return (f"TileConfig(wg_size={self.wg_size}, items={self.items}, "
-                f"max_slices={self.max_slices}, variant={self.variant})")
+        return (f"TileConfig(wg_size={self.wg_size}, items={self.items}, max_slices={self.max_slices}, "
+                f"variant={self.variant}, slice_steps={self.slice_steps}, chunk={self.chunk})")
+
+    def as_tuple(self):
+        return (self.wg_size, self.items, self.max_slices, self.variant, self.slice_steps, self.chunk)
+
+
+class ListDesc(ctypes.Structure):
+    _fields_ = [
+        ("data", ctypes.c_void_p),
+        ("removals", ctypes.c_void_p),
+        ("n", ctypes.c_int64),
+        ("k", ctypes.c_int64),
+        ("elem_bytes", ctypes.c_uint32),
+        ("method", ctypes.c_uint32),
+    ]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("nitems", ctypes.c_int64),
+        ("bitmap_bytes", ctypes.c_int64),
+        ("grid", ctypes.c_int32),
+        ("lists_active", ctypes.c_int32),
+    ]
 
 
 class Stats(ctypes.Structure):
@@ -68,7 +93,30 @@ SIGNATURES = [
     ("rz_last_stats", _i32, [_vp, ctypes.POINTER(Stats)]),
     ("rz_last_profile", _i32, [_vp, ctypes.POINTER(ctypes.c_float)]),
     ("rz_partition_on", _i32, [_vp, _vp, _i64, _u32, _vp, _i64, _u32, _vp]),
+    ("rz_batch_create", _i32, [_vp, ctypes.POINTER(ListDesc), _i32, _u32, ctypes.POINTER(TileConfig),
+                               ctypes.POINTER(_vp)]),
+    ("rz_batch_launch", _i32, [_vp, _vp]),
+    ("rz_batch_capture", _i32, [_vp, _vp]),
+    ("rz_batch_get_info", _i32, [_vp, ctypes.POINTER(BatchInfo)]),
+    ("rz_batch_destroy", _i32, [_vp]),
+    ("rz_remove_batched", _i32, [_vp, ctypes.POINTER(ListDesc), _i32, _u32, _vp]),
+    ("rz_gen_uniform_removals", _i32, [_vp, _vp, _i64, _i64, ctypes.c_uint64, _vp]),
+    ("rz_gen_payload", _i32, [_vp, _vp, _i64, _u32, ctypes.c_uint64, _vp]),
+    ("rz_tune_candidates", _i32, [_i32, ctypes.POINTER(TileConfig), _i32, ctypes.POINTER(_i32)]),
+    ("rz_tune", _i32, [_vp, _u32, _i64, _i64, ctypes.POINTER(TileConfig), _i32, _i32, _i32,
+                       ctypes.POINTER(TileConfig), ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    ("rz_tune_batch", _i32, [_vp, _u32, _i32, _i64, _i64, ctypes.c_double, _i32, ctypes.POINTER(TileConfig), _i32,
+                             ctypes.POINTER(TileConfig), ctypes.POINTER(ctypes.c_float),
+                             ctypes.POINTER(ctypes.c_float)]),
+    ("rz_table_get", _i32, [_vp, _u32, _i32, ctypes.POINTER(TileConfig)]),
+    ("rz_table_set", _i32, [_vp, _u32, _i32, ctypes.POINTER(TileConfig)]),
+    ("rz_batch_config_get", _i32, [_vp, ctypes.POINTER(TileConfig)]),
+    ("rz_batch_config_set", _i32, [_vp, ctypes.POINTER(TileConfig)]),
+    ("rz_table_save", _i32, [_vp, ctypes.c_char_p]),
+    ("rz_table_load", _i32, [_vp, ctypes.c_char_p]),
 ]
+
+TUNED_TABLE = os.path.join(_HERE, "tuned_gfx950.txt")
 
 
 def load():

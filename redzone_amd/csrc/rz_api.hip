@@ -54,6 +54,8 @@ rz_tile_config default_config(uint32_t esz, int64_t n, int64_t k) {
   c.items = k >= (1 << 22) ? 8 : 4;
   c.max_slices = 2048;
   c.variant = 0;
+  c.slice_steps = 4;
+  c.chunk = 1;
   return c;
 }
 
@@ -109,6 +111,9 @@ rz_status rz_ctx_create(rz_ctx** out, int32_t device) {
   memset(ctx->ctrl_host, 0, kCtrlWords * 4);
   for (int e = 0; e < 3; ++e)
     for (int b = 0; b < 33; ++b) ctx->table[e][b] = default_config(4u << e, 0, b ? (1ll << b) : 0);
+  ctx->batch_cfg = default_config(4, 0, 0);
+  ctx->batch_cfg.items = 8;
+  ctx->batch_cfg.slice_steps = 2;
   *out = ctx;
   return RZ_OK;
 }
@@ -117,7 +122,9 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
   if (!ctx) return RZ_OK;
   DeviceGuard g(ctx->device);
   (void)hipDeviceSynchronize();
-  DeviceBuffer* bufs[] = {&ctx->tail_bits, &ctx->list_bits, &ctx->counts, &ctx->orphans, &ctx->status, &ctx->ctrl};
+  if (ctx->last_batch) rz_batch_destroy(ctx->last_batch);
+  DeviceBuffer* bufs[] = {&ctx->tail_bits, &ctx->list_bits, &ctx->counts, &ctx->orphans,
+                          &ctx->status,    &ctx->ctrl,      &ctx->queue};
   for (DeviceBuffer* b : bufs)
     if (b->ptr) (void)hipFree(b->ptr);
   if (ctx->ctrl_host) (void)hipHostFree(ctx->ctrl_host);
@@ -201,6 +208,21 @@ rz_status rz_last_stats(rz_ctx* ctx, rz_stats* out) {
   out->orphan_pairs = (int64_t)(h[kCtrlStatOrphans] | ((uint64_t)h[kCtrlStatOrphans + 1] << 32));
   out->error_flags = (int32_t)h[kCtrlError];
   return RZ_OK;
+}
+
+rz_status rz_remove_batched(rz_ctx* ctx, const rz_list_desc* lists, int32_t nlists, uint32_t flags, void* stream) {
+  if (!ctx) return RZ_EINVAL;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  DeviceGuard g(ctx->device);
+  if (ctx->last_batch) {
+    rz_batch_destroy(ctx->last_batch);
+    ctx->last_batch = nullptr;
+  }
+  rz_batch* b = nullptr;
+  rz_status st = rz_batch_create(ctx, lists, nlists, flags, nullptr, &b);
+  if (st != RZ_OK) return st;
+  ctx->last_batch = b;
+  return rz_batch_launch(b, stream);
 }
 
 rz_status rz_last_profile(rz_ctx* ctx, float ms[4]) {

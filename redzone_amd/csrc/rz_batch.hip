@@ -1,0 +1,421 @@
+// rz_batch.hip -- batched Redzone: many lists in one persistent launch that
+// pulls work items from a device queue; also the single-launch form of the
+// plain path (one list, items computed from the ticket).
+//
+// Work items of a list with k removals cut into m slices of L pairs:
+//   SINGLE        (m == 1) one workgroup runs mark, move and pair for the
+//                 whole list (the one-block algorithm, RemoveFromList2,
+//                 redzone.hxx:236-444, on a 64-lane wavefront workgroup);
+//   MARK  j       marks of removals [jL, (j+1)L) into the list's tail bitmap;
+//   MOVE  j       slice j, after every MARK of the list has finished;
+//   PAIR  j       share j of the orphan pairs, after every MOVE has finished.
+// Items are dequeued in index order with an atomic ticket, and every wait is on
+// items with smaller indices (all MARK items precede all MOVE items, which
+// precede all PAIR items), so the queue drains whatever the number of
+// resident workgroups: no cooperative launch is needed.  The last workgroup to
+// leave resets the ticket and the per-list counters, so the next launch (or
+// graph replay) starts clean without a memset.
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "rz_redzone_dev.hpp"
+
+namespace rz {
+
+constexpr int kQueueMaxSlices = 1024;  // slices per list in the queue kernel (LDS prefix)
+enum ItemKind : uint32_t { kSingle = 0, kMark = 1, kMoveItem = 2, kPair = 3 };
+
+struct ListPlan {
+  uint64_t data;      // T*
+  uint64_t R;         // int32_t* removal list
+  uint64_t Rd;        // int32_t* dest-orphan storage (R, or preserve scratch)
+  uint32_t n, k, sZ, L;
+  uint32_t m, esz;
+  uint32_t bits_off;    // first bitmap word of this list
+  uint32_t counts_off;  // first slice-count word of this list
+  uint32_t npair;       // number of PAIR items
+  uint32_t pad;
+};
+static_assert(sizeof(ListPlan) == 64, "ListPlan is one 64-byte record");
+
+// queue control words (uint32 index into the queue control block)
+enum QueueWord : int { kQTicket = 0, kQExit = 32, kQWords = 64 };
+
+template <typename T, int WG, int IPT>
+__device__ __forceinline__ void run_item(const ListPlan& P, uint32_t li, uint32_t kind, uint32_t j,
+                                         uint32_t* __restrict__ done, uint32_t* __restrict__ bits,
+                                         uint64_t* __restrict__ counts, uint32_t* __restrict__ ctrl,
+                                         MoveShared<WG, IPT>& msh, PairShared<WG, kQueueMaxSlices>& psh) {
+  T* A = (T*)P.data;
+  const int32_t* R = (const int32_t*)P.R;
+  int32_t* Rd = (int32_t*)P.Rd;
+  uint32_t* lb = bits + P.bits_off;
+  uint64_t* lc = counts + P.counts_off;
+  const uint32_t tid = threadIdx.x;
+  if (kind == kSingle) {
+    mark_body<false>(R, P.k, P.k, P.sZ, lb, nullptr, tid, WG);
+    wait_all_memory();
+    __syncthreads();
+    move_slice<T, WG, IPT, true>(A, R, Rd, P.k, P.sZ, P.L, 0, lb, lc, msh);
+    wait_all_memory();
+    __syncthreads();
+    pair_body<T, WG, kQueueMaxSlices, true>(A, Rd, P.sZ, P.L, 1, lc, ctrl, 0, 0, 1, psh);
+  } else if (kind == kMark) {
+    const uint32_t a = j * P.L, b = min(a + P.L, P.k);
+    mark_body<false>(R + a, b - a, P.k, P.sZ, lb, nullptr, tid, WG);
+    signal_done(&done[2 * li]);
+  } else if (kind == kMoveItem) {
+    wait_count(&done[2 * li], P.m, ctrl);
+    move_slice<T, WG, IPT, true>(A, R, Rd, P.k, P.sZ, P.L, j, lb, lc, msh, &done[2 * li + 1]);
+  } else {
+    wait_count(&done[2 * li + 1], P.m, ctrl);
+    pair_body<T, WG, kQueueMaxSlices, true>(A, Rd, P.sZ, P.L, P.m, lc, ctrl, 0, j, P.npair, psh);
+  }
+}
+
+// items == nullptr: one list (`single`), item q is MARK q (q < m), MOVE q-m
+// (q < 2m) or PAIR q-2m; with m == 1 the only item is SINGLE.
+template <int WG, int IPT>
+__global__ __launch_bounds__(WG) void rz_queue(ListPlan single, const ListPlan* __restrict__ plans,
+                                                const uint2* __restrict__ items, uint32_t nitems, uint32_t chunk,
+                                                uint32_t* __restrict__ qctl, uint32_t* __restrict__ done,
+                                                uint32_t nlists, uint32_t* __restrict__ bits,
+                                                uint64_t* __restrict__ counts, uint32_t* __restrict__ ctrl) {
+  __shared__ MoveShared<WG, IPT> msh;
+  __shared__ PairShared<WG, kQueueMaxSlices> psh;
+  __shared__ uint32_t s_q, s_last;
+  const uint32_t tid = threadIdx.x;
+  for (;;) {
+    if (tid == 0) s_q = add_relaxed(&qctl[kQTicket], chunk);
+    __syncthreads();
+    const uint32_t q0 = s_q;
+    __syncthreads();
+    if (q0 >= nitems) break;
+    const uint32_t q1 = min(q0 + chunk, nitems);
+    for (uint32_t q = q0; q < q1; ++q) {
+      uint32_t li, kind, j;
+      ListPlan P;
+      if (items) {
+        const uint2 it = items[q];
+        li = it.x & 0x0fffffffu;
+        kind = it.x >> 28;
+        j = it.y;
+        P = plans[li];
+      } else {
+        P = single;
+        li = 0;
+        if (P.m == 1) {
+          kind = kSingle;
+          j = 0;
+        } else if (q < P.m) {
+          kind = kMark;
+          j = q;
+        } else if (q < 2 * P.m) {
+          kind = kMoveItem;
+          j = q - P.m;
+        } else {
+          kind = kPair;
+          j = q - 2 * P.m;
+        }
+      }
+      if (P.esz == 4) run_item<uint32_t, WG, IPT>(P, li, kind, j, done, bits, counts, ctrl, msh, psh);
+      else if (P.esz == 8) run_item<uint64_t, WG, IPT>(P, li, kind, j, done, bits, counts, ctrl, msh, psh);
+      else run_item<uint4, WG, IPT>(P, li, kind, j, done, bits, counts, ctrl, msh, psh);
+    }
+  }
+  // the last workgroup out resets the queue state for the next launch
+  if (tid == 0) {
+    wait_all_memory();
+    s_last = add_relaxed(&qctl[kQExit], 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (s_last) {
+    for (uint32_t i = tid; i < 2 * nlists; i += WG) st_relaxed(&done[i], 0u);
+    if (tid == 0) {
+      st_relaxed(&qctl[kQTicket], 0u);
+      st_relaxed(&qctl[kQExit], 0u);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+typedef void (*QueueKernel)(ListPlan, const ListPlan*, const uint2*, uint32_t, uint32_t, uint32_t*, uint32_t*,
+                            uint32_t, uint32_t*, uint64_t*, uint32_t*);
+
+static QueueKernel queue_kernel(int wg, int ipt) {
+  if (wg == 256 && ipt == 4) return rz_queue<256, 4>;
+  if (wg == 256 && ipt == 8) return rz_queue<256, 8>;
+  if (wg == 512 && ipt == 4) return rz_queue<512, 4>;
+  if (wg == 512 && ipt == 8) return rz_queue<512, 8>;
+  return nullptr;
+}
+
+static rz_status queue_grid(rz_ctx* ctx, QueueKernel kern, int wg, uint32_t nitems, uint32_t* grid) {
+  int per_cu = 0;
+  RZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, wg, 0));
+  if (per_cu < 1) per_cu = 1;
+  uint32_t g = (uint32_t)per_cu * (uint32_t)ctx->cu_count;
+  if (g > nitems) g = nitems;
+  *grid = g < 1 ? 1 : g;
+  return RZ_OK;
+}
+
+// Slice length of one list in the queue kernel: a multiple of one step, at
+// least cfg.max_slices-derived length, and at most kQueueMaxSlices slices.
+static void queue_slices(int64_t k, const rz_tile_config& cfg, uint32_t* L, uint32_t* m) {
+  const int64_t step = (int64_t)cfg.wg_size * cfg.items;
+  const int64_t steps = cfg.slice_steps > 0 ? cfg.slice_steps : 4;
+  int64_t len = steps * step;
+  const int64_t min_len = (k + kQueueMaxSlices - 1) / kQueueMaxSlices;
+  if (len < min_len) len = (min_len + step - 1) / step * step;
+  *L = (uint32_t)len;
+  *m = (uint32_t)((k + len - 1) / len);
+}
+
+static void fill_plan(ListPlan& P, void* data, int32_t* R, int32_t* Rd, int64_t n, int64_t k, uint32_t esz,
+                      const rz_tile_config& cfg) {
+  memset(&P, 0, sizeof(P));
+  P.data = (uint64_t)(uintptr_t)data;
+  P.R = (uint64_t)(uintptr_t)R;
+  P.Rd = (uint64_t)(uintptr_t)Rd;
+  P.n = (uint32_t)n;
+  P.k = (uint32_t)k;
+  P.sZ = (uint32_t)(n - k);
+  P.esz = esz;
+  queue_slices(k, cfg, &P.L, &P.m);
+  P.npair = P.m == 1 ? 1 : std::max<uint32_t>(1, (P.m + 3) / 4);
+}
+
+// Single list through the queue kernel (plain-path variant 2).
+rz_status launch_queue_single(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, int32_t* R, int64_t k,
+                              uint32_t flags, const rz_tile_config& cfg, hipStream_t s) {
+  QueueKernel kern = queue_kernel(cfg.wg_size, cfg.items);
+  if (!kern) return RZ_EINVAL;
+  rz_status st;
+  int32_t* Rd = R;
+  if (flags & RZ_PRESERVE_REMOVALS) {
+    if ((st = ensure(ctx, ctx->orphans, (size_t)k * 4, s)) != RZ_OK) return st;
+    Rd = (int32_t*)ctx->orphans.ptr;
+  }
+  ListPlan P;
+  fill_plan(P, data, R, Rd, n, k, esz, cfg);
+  if ((st = ensure(ctx, ctx->tail_bits, (size_t)((k + 63) / 64) * 8, s)) != RZ_OK) return st;
+  if ((st = ensure(ctx, ctx->counts, (size_t)P.m * 8, s)) != RZ_OK) return st;
+  if ((st = ensure(ctx, ctx->queue, 4096, s)) != RZ_OK) return st;
+  const uint32_t nitems = P.m == 1 ? 1 : 2 * P.m + P.npair;
+  uint32_t grid;
+  if ((st = queue_grid(ctx, kern, cfg.wg_size, nitems, &grid)) != RZ_OK) return st;
+  uint32_t* qctl = (uint32_t*)ctx->queue.ptr;
+  if ((st = prof_mark(ctx, flags, 0, s)) != RZ_OK) return st;
+  if ((st = prof_mark(ctx, flags, 1, s)) != RZ_OK) return st;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(cfg.wg_size), 0, s, P, (const ListPlan*)nullptr, (const uint2*)nullptr,
+                     nitems, 1u, qctl, qctl + kQWords, 1u, (uint32_t*)ctx->tail_bits.ptr,
+                     (uint64_t*)ctx->counts.ptr, (uint32_t*)ctx->ctrl.ptr);
+  RZ_HIP(hipGetLastError());
+  if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;
+  return prof_mark(ctx, flags, 3, s);
+}
+
+}  // namespace rz
+
+// ---------------------------------------------------------------------------
+// batch objects
+// ---------------------------------------------------------------------------
+struct rz_batch {
+  rz_ctx* ctx = nullptr;
+  int32_t nlists = 0;
+  rz_tile_config cfg;
+  uint32_t flags = 0;
+  uint32_t nitems = 0, grid = 0, chunk = 1;
+  rz::QueueKernel kern = nullptr;
+  rz::DeviceBuffer plans, items, qctl, bits, counts, orphans;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  rz_batch_info info;
+};
+
+using namespace rz;
+
+static void free_buf(DeviceBuffer& b) {
+  if (b.ptr) (void)hipFree(b.ptr);
+  b.ptr = nullptr;
+  b.bytes = 0;
+}
+
+extern "C" {
+
+rz_status rz_batch_create(rz_ctx* ctx, const rz_list_desc* lists, int32_t nlists, uint32_t flags,
+                          const rz_tile_config* cfg_in, rz_batch** out) {
+  if (!ctx || !out || nlists < 0 || (nlists > 0 && !lists)) return RZ_EINVAL;
+  if (nlists >= (1 << 28)) return RZ_EINVAL;
+  *out = nullptr;
+  rz_tile_config cfg = cfg_in ? *cfg_in : ctx->batch_cfg;
+  QueueKernel kern = queue_kernel(cfg.wg_size, cfg.items);
+  if (!kern) return RZ_EINVAL;
+  int dev_prev = -1;
+  (void)hipGetDevice(&dev_prev);
+  if (dev_prev != ctx->device) (void)hipSetDevice(ctx->device);
+
+  std::vector<ListPlan> plans((size_t)nlists);
+  std::vector<uint64_t> keys;  // (k desc, index) for LPT ordering
+  uint64_t bits_words = 0, count_words = 0, orphan_words = 0;
+  rz_status st = RZ_OK;
+  for (int32_t i = 0; i < nlists && st == RZ_OK; ++i) {
+    const rz_list_desc& d = lists[i];
+    if (d.n < 0 || d.k < 0 || d.k > d.n || d.n >= (1ll << 31) || esz_index(d.elem_bytes) < 0 ||
+        (d.k > 0 && (!d.data || !d.removals))) {
+      st = RZ_EINVAL;
+      break;
+    }
+    fill_plan(plans[i], d.data, d.removals, d.removals, d.n, d.k, d.elem_bytes, cfg);
+    plans[i].bits_off = (uint32_t)bits_words;
+    plans[i].counts_off = (uint32_t)count_words;
+    if (flags & RZ_PRESERVE_REMOVALS) plans[i].Rd = orphan_words;  // offset, patched after allocation
+    if (d.k > 0 && d.k < d.n) {
+      bits_words += (uint64_t)((d.k + 63) / 64) * 2;
+      count_words += plans[i].m;
+      orphan_words += (uint64_t)d.k;
+      keys.push_back(((uint64_t)(0xffffffffu - (uint32_t)d.k) << 32) | (uint32_t)i);
+    } else {
+      plans[i].m = 0;  // nothing to do
+    }
+  }
+  if (st != RZ_OK) {
+    if (dev_prev >= 0 && dev_prev != ctx->device) (void)hipSetDevice(dev_prev);
+    return st;
+  }
+  if (bits_words >= (1ull << 32) || count_words >= (1ull << 32)) return RZ_EINVAL;
+  std::sort(keys.begin(), keys.end());
+  std::vector<uint2> items;
+  auto push = [&](uint32_t li, uint32_t kind, uint32_t j) { items.push_back(make_uint2(li | (kind << 28), j)); };
+  for (uint64_t key : keys) {
+    const uint32_t li = (uint32_t)key;
+    if (plans[li].m > 1)
+      for (uint32_t j = 0; j < plans[li].m; ++j) push(li, kMark, j);
+  }
+  for (uint64_t key : keys) {
+    const uint32_t li = (uint32_t)key;
+    if (plans[li].m == 1) push(li, kSingle, 0);
+  }
+  for (uint64_t key : keys) {
+    const uint32_t li = (uint32_t)key;
+    if (plans[li].m > 1)
+      for (uint32_t j = 0; j < plans[li].m; ++j) push(li, kMoveItem, j);
+  }
+  for (uint64_t key : keys) {
+    const uint32_t li = (uint32_t)key;
+    if (plans[li].m > 1)
+      for (uint32_t j = 0; j < plans[li].npair; ++j) push(li, kPair, j);
+  }
+
+  rz_batch* b = new rz_batch();
+  b->ctx = ctx;
+  b->nlists = nlists;
+  b->cfg = cfg;
+  b->flags = flags;
+  b->kern = kern;
+  b->nitems = (uint32_t)items.size();
+  b->chunk = cfg.chunk > 0 ? (uint32_t)cfg.chunk : 1u;
+  hipStream_t s0 = nullptr;
+  if ((st = ensure(ctx, b->plans, std::max<size_t>(64, plans.size() * sizeof(ListPlan)), s0)) != RZ_OK ||
+      (st = ensure(ctx, b->items, std::max<size_t>(8, items.size() * sizeof(uint2)), s0)) != RZ_OK ||
+      (st = ensure(ctx, b->qctl, (size_t)(kQWords + 2 * (size_t)nlists + 64) * 4, s0)) != RZ_OK ||
+      (st = ensure(ctx, b->bits, std::max<size_t>(8, (size_t)bits_words * 4), s0)) != RZ_OK ||
+      (st = ensure(ctx, b->counts, std::max<size_t>(8, (size_t)count_words * 8), s0)) != RZ_OK ||
+      ((flags & RZ_PRESERVE_REMOVALS) &&
+       (st = ensure(ctx, b->orphans, std::max<size_t>(8, (size_t)orphan_words * 4), s0)) != RZ_OK)) {
+    rz_batch_destroy(b);
+    return st;
+  }
+  if (flags & RZ_PRESERVE_REMOVALS)
+    for (auto& P : plans) P.Rd = (uint64_t)(uintptr_t)((int32_t*)b->orphans.ptr + P.Rd);
+  if (!plans.empty() && hipMemcpy(b->plans.ptr, plans.data(), plans.size() * sizeof(ListPlan),
+                                  hipMemcpyHostToDevice) != hipSuccess) {
+    rz_batch_destroy(b);
+    return RZ_EHIP;
+  }
+  if (!items.empty() &&
+      hipMemcpy(b->items.ptr, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice) != hipSuccess) {
+    rz_batch_destroy(b);
+    return RZ_EHIP;
+  }
+  if ((st = queue_grid(ctx, kern, cfg.wg_size, b->nitems, &b->grid)) != RZ_OK) {
+    rz_batch_destroy(b);
+    return st;
+  }
+  b->info.nitems = b->nitems;
+  b->info.grid = b->grid;
+  b->info.bitmap_bytes = (int64_t)bits_words * 4;
+  b->info.lists_active = (int32_t)keys.size();
+  if (dev_prev >= 0 && dev_prev != ctx->device) (void)hipSetDevice(dev_prev);
+  *out = b;
+  return RZ_OK;
+}
+
+rz_status rz_batch_launch(rz_batch* b, void* stream) {
+  if (!b) return RZ_EINVAL;
+  if (b->nitems == 0) return RZ_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (b->exec) {
+    RZ_HIP(hipGraphLaunch(b->exec, s));
+    return RZ_OK;
+  }
+  uint32_t* q = (uint32_t*)b->qctl.ptr;
+  hipLaunchKernelGGL(b->kern, dim3(b->grid), dim3(b->cfg.wg_size), 0, s, ListPlan{},
+                     (const ListPlan*)b->plans.ptr, (const uint2*)b->items.ptr, b->nitems, b->chunk, q,
+                     q + kQWords, (uint32_t)b->nlists, (uint32_t*)b->bits.ptr, (uint64_t*)b->counts.ptr,
+                     (uint32_t*)b->ctx->ctrl.ptr);
+  RZ_HIP(hipGetLastError());
+  return RZ_OK;
+}
+
+rz_status rz_batch_capture(rz_batch* b, void* stream) {
+  if (!b) return RZ_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  RZ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  hipLaunchKernelGGL(b->kern, dim3(b->grid), dim3(b->cfg.wg_size), 0, s, ListPlan{},
+                     (const ListPlan*)b->plans.ptr, (const uint2*)b->items.ptr, b->nitems, b->chunk,
+                     (uint32_t*)b->qctl.ptr, (uint32_t*)b->qctl.ptr + kQWords, (uint32_t)b->nlists,
+                     (uint32_t*)b->bits.ptr, (uint64_t*)b->counts.ptr, (uint32_t*)b->ctx->ctrl.ptr);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &g);
+  if (e != hipSuccess || !g) return RZ_EHIP;
+  b->graph = g;
+  RZ_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  return RZ_OK;
+}
+
+rz_status rz_batch_get_info(rz_batch* b, rz_batch_info* out) {
+  if (!b || !out) return RZ_EINVAL;
+  *out = b->info;
+  return RZ_OK;
+}
+
+rz_status rz_batch_destroy(rz_batch* b) {
+  if (!b) return RZ_OK;
+  (void)hipDeviceSynchronize();
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  free_buf(b->plans);
+  free_buf(b->items);
+  free_buf(b->qctl);
+  free_buf(b->bits);
+  free_buf(b->counts);
+  free_buf(b->orphans);
+  delete b;
+  return RZ_OK;
+}
+
+}  // extern "C"

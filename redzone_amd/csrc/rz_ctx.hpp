@@ -49,6 +49,9 @@ struct rz_ctx {
   rz::DeviceBuffer orphans;    // dest orphans when RZ_PRESERVE_REMOVALS is set
   rz::DeviceBuffer status;     // look-back status words of the partition
   rz::DeviceBuffer ctrl;       // control block (rz::CtrlWord)
+  rz::DeviceBuffer queue;      // ticket / exit / done words of the single-list queue launch
+  rz_tile_config batch_cfg;    // default launch shape of batches
+  rz_batch* last_batch = nullptr;  // plan of the last rz_remove_batched call
   uint32_t* ctrl_host = nullptr;  // pinned mirror for stats / validation reads
   rz_tile_config table[3][33];    // [element size 4/8/16][floor(log2 k)]
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // RZ_PROFILE marks
@@ -73,6 +76,8 @@ rz_status launch_plain(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, int32_t
 rz_status launch_partition(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, const int32_t* R, int64_t k,
                            uint32_t flags, hipStream_t s);
 rz_status launch_validate(rz_ctx* ctx, const int32_t* R, int64_t n, int64_t k, hipStream_t s);
+rz_status launch_queue_single(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, int32_t* R, int64_t k,
+                              uint32_t flags, const rz_tile_config& cfg, hipStream_t s);
 // RZ_PROFILE: record mark i (0..3) on the stream when profiling
 rz_status prof_mark(rz_ctx* ctx, uint32_t flags, int i, hipStream_t s);
 // fill `count` uint32 words with `v` by a kernel (graph-capture safe, unlike hipMemsetAsync)

@@ -1,206 +1,44 @@
 // rz_plain.hip -- Redzone O(k) removal from one list on gfx950.
 //
 // The removal list R (k entries) pairs with the red zone Z = A[n-k, n): pair i
-// moves Z[i] into hole R[i].  Three dependent steps, each a kernel on the
-// caller's stream (the kernel boundary is the grid-wide barrier; a regular
-// launch on MI355X costs ~2 us, a cooperative launch ~20 us, and an in-kernel
-// barrier that fences at agent scope ~14 us at 256 workgroups):
+// moves Z[i] into hole R[i].  Three dependent steps (bodies in
+// rz_redzone_dev.hpp):
 //
-//   1. rz_mark_tail   every R[i] >= n-k kills tail item R[i]-(n-k): one bit in
-//                     a k-bit tail bitmap (relaxed atomicOr).  Replaces the
-//                     sentinel write of redzone.hxx:698-707, so no payload
-//                     value is reserved.
-//   2. rz_move        R is cut into slices of L pairs, one workgroup each.  A
-//                     workgroup loads its R, Z and bitmap words coalesced,
-//                     classifies every pair with two 64-lane ballots, and ranks
-//                     the orphans of its slice in i order with one LDS scan per
-//                     step (redzone.hxx:714-822 does two block scans per step
-//                     on 32-lane warps).  Moves go straight to their holes;
-//                     dest orphans go to the slice's own part of R, source
-//                     orphans to the slice's own part of Z.  The slice's
-//                     bitmap words are cleared after use, so the bitmap is
-//                     zero again for the next call without a memset.
-//   3. rz_pair        every workgroup scans the per-slice orphan counts in LDS
-//                     and pairs the t-th dest orphan with the t-th source
-//                     orphan (redzone.hxx:834-934).  Orphans are ordered by
-//                     slice then by i, so the result is the canonical one for
-//                     every launch shape.
+//   1. mark   every R[i] >= n-k kills tail item R[i]-(n-k): one bit in a k-bit
+//             tail bitmap (relaxed atomicOr).  Replaces the sentinel write of
+//             redzone.hxx:698-707, so no payload value is reserved.
+//   2. move   R is cut into slices of L pairs, one workgroup each.  A
+//             workgroup loads its R, Z and bitmap words coalesced, classifies
+//             every pair with two 64-lane ballots, and ranks the orphans of its
+//             slice in i order with one LDS scan per step (redzone.hxx:714-822
+//             does two block scans per step on 32-lane warps).  Moves go
+//             straight to their holes; dest orphans go to the slice's own part
+//             of R, source orphans to the slice's own part of Z.  The slice's
+//             bitmap words are cleared after use, so the bitmap is zero again
+//             for the next call without a memset.
+//   3. pair   every workgroup scans the per-slice orphan counts in LDS and
+//             pairs the t-th dest orphan with the t-th source orphan
+//             (redzone.hxx:834-934).  Orphans are ordered by slice then by i,
+//             so the result is the canonical one for every launch shape.
+//
+// Launch variants: (0) one kernel per step on the caller's stream, the kernel
+// boundary being the grid-wide barrier; (1) one cooperative kernel with
+// in-kernel barriers made of relaxed atomics (RemoveFromList_GPUWide's
+// structure, redzone.hxx:626-941); (2) the persistent work-queue kernel of
+// rz_batch.hip with a single list.
 //
 // Contract violations (duplicates, out-of-range indices) never cause an
 // out-of-bounds access: marks are taken only for indices inside the tail,
 // moves only into holes below n-k, and pairing stops at the smaller orphan
 // total (a mismatch sets an error bit in the control block).
-#include "rz_ctx.hpp"
-#include "rz_device.hpp"
+#include "rz_redzone_dev.hpp"
 
 namespace rz {
-
-// ---------------------------------------------------------------------------
-// coherent element access (used where another workgroup of the same kernel
-// reads the value: the cooperative single-launch variant)
-// ---------------------------------------------------------------------------
-template <typename T> __device__ __forceinline__ void st_coh(T* p, T v);
-template <> __device__ __forceinline__ void st_coh<uint32_t>(uint32_t* p, uint32_t v) { st_relaxed(p, v); }
-template <> __device__ __forceinline__ void st_coh<uint64_t>(uint64_t* p, uint64_t v) { st_relaxed(p, v); }
-template <> __device__ __forceinline__ void st_coh<uint4>(uint4* p, uint4 v) {
-  uint64_t* q = (uint64_t*)p;
-  st_relaxed(q, (uint64_t)v.x | ((uint64_t)v.y << 32));
-  st_relaxed(q + 1, (uint64_t)v.z | ((uint64_t)v.w << 32));
-}
-template <typename T> __device__ __forceinline__ T ld_coh(const T* p);
-template <> __device__ __forceinline__ uint32_t ld_coh<uint32_t>(const uint32_t* p) { return ld_relaxed(p); }
-template <> __device__ __forceinline__ uint64_t ld_coh<uint64_t>(const uint64_t* p) { return ld_relaxed(p); }
-template <> __device__ __forceinline__ uint4 ld_coh<uint4>(const uint4* p) {
-  const uint64_t* q = (const uint64_t*)p;
-  const uint64_t lo = ld_relaxed(q), hi = ld_relaxed(q + 1);
-  return make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
-}
-template <bool COH, typename T> __device__ __forceinline__ void st_x(T* p, T v) {
-  if constexpr (COH) st_coh<T>(p, v); else *p = v;
-}
-template <bool COH, typename T> __device__ __forceinline__ T ld_x(const T* p) {
-  if constexpr (COH) return ld_coh<T>(p); else return *p;
-}
-
-// ---------------------------------------------------------------------------
-// 1. mark
-// ---------------------------------------------------------------------------
-template <bool STATS>
-__device__ __forceinline__ void mark_body(const int32_t* __restrict__ R, uint32_t k, uint32_t sZ,
-                                          uint32_t* __restrict__ bits, uint64_t* __restrict__ stat_tail,
-                                          uint32_t gtid, uint32_t stride) {
-  constexpr int U = 4;
-  uint32_t hits = 0;
-  for (uint32_t i0 = gtid; i0 < k; i0 += stride * U) {
-    uint32_t r[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t i = i0 + u * stride;
-      r[u] = i < k ? (uint32_t)R[i] : sZ - 1u;  // sZ-1 is never inside the tail (k > 0 => sZ < n)
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t z = r[u] - sZ;  // wraps to >= 2^31 > k below the tail
-      if (z < k) {
-        or_relaxed(&bits[z >> 5], 1u << (z & 31));
-        if (STATS) ++hits;
-      }
-    }
-  }
-  if (STATS) {
-    hits = wave_sum(hits);
-    if (lane_id() == 0 && hits) add_relaxed(stat_tail, (uint64_t)hits);
-  }
-}
 
 template <bool STATS>
 __global__ __launch_bounds__(256) void rz_mark_tail(const int32_t* __restrict__ R, uint32_t k, uint32_t sZ,
                                                      uint32_t* __restrict__ bits, uint64_t* __restrict__ stat_tail) {
-  mark_body<STATS>(R, k, sZ, bits, stat_tail, blockIdx.x * 256u + threadIdx.x, gridDim.x * 256u);
-}
-
-// ---------------------------------------------------------------------------
-// 2. move and collect orphans
-// ---------------------------------------------------------------------------
-// Item classes.  A pair with a dead source and an invalid destination does nothing.
-enum : uint32_t { kNone = 0, kMove = 1, kSrcOrphan = 2, kDstOrphan = 3 };
-
-template <int WG, int IPT>
-struct MoveShared {
-  static constexpr int E = IPT * (WG / kWave);
-  uint32_t off[E + 1];
-};
-
-// One slice [a, b) of pairs.  COH: bitmap read and orphans / counts written
-// coherently for readers in other workgroups of the same kernel.
-template <typename T, int WG, int IPT, bool COH>
-__device__ __forceinline__ void move_slice(T* __restrict__ A, const int32_t* __restrict__ R, int32_t* __restrict__ Rd,
-                                           uint32_t k, uint32_t sZ, uint32_t L, uint32_t slice,
-                                           uint32_t* __restrict__ bits, uint64_t* __restrict__ counts,
-                                           MoveShared<WG, IPT>& sh) {
-  constexpr int NW = WG / kWave;
-  constexpr int E = MoveShared<WG, IPT>::E;  // (step item, wave) count cells, in i order
-  constexpr int PER = (E + kWave - 1) / kWave;
-  uint32_t* s_off = sh.off;
-  const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
-  const uint32_t a = slice * L;
-  const uint32_t b = min(a + L, k);
-  T* __restrict__ Z = A + sZ;
-  uint32_t carry_s = 0, carry_d = 0;
-
-  for (uint32_t base = a; base < b; base += WG * IPT) {
-    uint32_t d[IPT], wv[IPT], info[IPT];
-    T z[IPT];
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const uint32_t p = base + j * WG + tid;
-      if (p < b) {
-        d[j] = (uint32_t)R[p];
-        z[j] = Z[p];
-        wv[j] = ld_x<COH>(&bits[p >> 5]);
-      } else {
-        d[j] = sZ;            // invalid destination
-        wv[j] = 0xffffffffu;  // dead source
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const uint32_t p = base + j * WG + tid;
-      const bool dead = (wv[j] >> (p & 31)) & 1u;
-      const bool dst_ok = d[j] < sZ;
-      const uint32_t cls = dead ? (dst_ok ? kDstOrphan : kNone) : (dst_ok ? kMove : kSrcOrphan);
-      const uint64_t ms = ballot(cls == kSrcOrphan);
-      const uint64_t md = ballot(cls == kDstOrphan);
-      info[j] = cls | (lane_rank(ms) << 8) | (lane_rank(md) << 16);
-      if (lane == 0) s_off[j * NW + wave] = (uint32_t)__popcll(ms) | ((uint32_t)__popcll(md) << 16);
-    }
-    // Every Z and R load of this step must have returned before any thread
-    // writes an orphan into this slice's own Z / R range.
-    wait_all_memory();
-    __syncthreads();
-    if (wave == 0) {
-      uint32_t loc[PER];
-      uint32_t sum = 0;
-#pragma unroll
-      for (int q = 0; q < PER; ++q) {
-        const int idx = lane * PER + q;
-        loc[q] = idx < E ? s_off[idx] : 0u;
-        sum += loc[q];
-      }
-      const uint32_t x = wave_incl_scan(sum);
-      uint32_t ex = x - sum;
-#pragma unroll
-      for (int q = 0; q < PER; ++q) {
-        const int idx = lane * PER + q;
-        if (idx < E) s_off[idx] = ex;
-        ex += loc[q];
-      }
-      if (lane == kWave - 1) s_off[E] = x;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const uint32_t cls = info[j] & 0xffu;
-      if (cls == kNone) continue;
-      if (cls == kMove) {
-        A[d[j]] = z[j];
-      } else {
-        const uint32_t off = s_off[j * NW + wave];
-        if (cls == kSrcOrphan) {
-          st_x<COH>(&Z[a + carry_s + (off & 0xffffu) + ((info[j] >> 8) & 0xffu)], z[j]);
-        } else {
-          st_x<COH>((uint32_t*)&Rd[a + carry_d + (off >> 16) + (info[j] >> 16)], d[j]);
-        }
-      }
-    }
-    const uint32_t tot = s_off[E];
-    carry_s += tot & 0xffffu;
-    carry_d += tot >> 16;
-    __syncthreads();
-  }
-  // leave the bitmap zero for the next call (slices are word aligned)
-  for (uint32_t w = (a >> 5) + tid; w < ((b + 31u) >> 5); w += WG) bits[w] = 0u;
-  if (tid == 0) st_x<COH>(&counts[slice], (uint64_t)carry_d | ((uint64_t)carry_s << 32));
+  mark_body<STATS>(R, k, k, sZ, bits, stat_tail, blockIdx.x * 256u + threadIdx.x, gridDim.x * 256u);
 }
 
 template <typename T, int WG, int IPT>
@@ -211,64 +49,12 @@ __global__ __launch_bounds__(WG) void rz_move(T* __restrict__ A, const int32_t* 
   move_slice<T, WG, IPT, false>(A, R, Rd, k, sZ, L, blockIdx.x, bits, counts, sh);
 }
 
-// ---------------------------------------------------------------------------
-// 3. pair orphans
-// ---------------------------------------------------------------------------
-template <int WG>
-struct PairShared {
-  uint64_t pre[kMaxSlices + 1];
-  uint64_t tmp[WG / kWave + 1];
-};
-
-// Exclusive prefix of the per-slice counts into LDS, then pairs t = start,
-// start+stride, ...  Dest orphans are the low 32 bits of each count word.
-template <typename T, int WG, bool COH>
-__device__ __forceinline__ void pair_body(T* __restrict__ A, const int32_t* __restrict__ Rd, uint32_t sZ,
-                                          uint32_t L, uint32_t ns, const uint64_t* __restrict__ counts,
-                                          uint32_t* __restrict__ ctrl, int stats, uint32_t block,
-                                          uint32_t nblocks, PairShared<WG>& sh) {
-  constexpr int PER = kMaxSlices / WG;
-  uint64_t* s_pre = sh.pre;
-  const int tid = threadIdx.x;
-  uint64_t loc[PER];
-  uint64_t sum = 0;
-#pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    const uint32_t s = tid * PER + q;
-    loc[q] = s < ns ? ld_x<COH>(&counts[s]) : 0ull;
-    sum += loc[q];
-  }
-  uint64_t total;
-  uint64_t ex = block_excl_scan<WG>(sum, sh.tmp, &total);
-#pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    const uint32_t s = tid * PER + q;
-    if (s < ns) s_pre[s] = ex;
-    ex += loc[q];
-  }
-  if (tid == 0) s_pre[ns] = total;
-  __syncthreads();
-  const uint32_t tot_d = (uint32_t)total, tot_s = (uint32_t)(total >> 32);
-  const uint32_t pairs = min(tot_d, tot_s);
-  if (block == 0 && tid == 0) {
-    if (tot_d != tot_s) or_relaxed(&ctrl[kCtrlError], kErrOrphanMismatch);
-    if (stats) st_relaxed((uint64_t*)&ctrl[kCtrlStatOrphans], (uint64_t)pairs);
-  }
-  T* __restrict__ Z = A + sZ;
-  for (uint32_t t = block * WG + tid; t < pairs; t += nblocks * WG) {
-    const int sd = upper_row((int)ns, t, [&](int s) { return (uint32_t)s_pre[s]; });
-    const int ss = upper_row((int)ns, t, [&](int s) { return (uint32_t)(s_pre[s] >> 32); });
-    const uint32_t dst = ld_x<COH>((const uint32_t*)&Rd[sd * L + (t - (uint32_t)s_pre[sd])]);
-    A[dst] = ld_x<COH>(&Z[ss * L + (t - (uint32_t)(s_pre[ss] >> 32))]);
-  }
-}
-
 template <typename T>
 __global__ __launch_bounds__(256) void rz_pair(T* __restrict__ A, const int32_t* __restrict__ Rd, uint32_t sZ,
                                                 uint32_t L, uint32_t ns, const uint64_t* __restrict__ counts,
                                                 uint32_t* __restrict__ ctrl, int stats) {
-  __shared__ PairShared<256> sh;
-  pair_body<T, 256, false>(A, Rd, sZ, L, ns, counts, ctrl, stats, blockIdx.x, gridDim.x, sh);
+  __shared__ PairShared<256, kMaxSlices> sh;
+  pair_body<T, 256, kMaxSlices, false>(A, Rd, sZ, L, ns, counts, ctrl, stats, blockIdx.x, gridDim.x, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -301,24 +87,103 @@ __global__ __launch_bounds__(WG) void rz_coop(T* __restrict__ A, const int32_t* 
                                                uint64_t* __restrict__ counts, uint32_t* __restrict__ ctrl,
                                                int stats) {
   __shared__ MoveShared<WG, IPT> msh;
-  __shared__ PairShared<WG> psh;
+  __shared__ PairShared<WG, kMaxSlices> psh;
   const uint64_t base = ld_relaxed((const uint64_t*)&ctrl[kCtrlBarEpoch]);
   const uint32_t G = gridDim.x;
   if (stats)
-    mark_body<true>(R, k, sZ, bits, (uint64_t*)&ctrl[kCtrlStatTail], blockIdx.x * WG + threadIdx.x, G * WG);
+    mark_body<true>(R, k, k, sZ, bits, (uint64_t*)&ctrl[kCtrlStatTail], blockIdx.x * WG + threadIdx.x, G * WG);
   else
-    mark_body<false>(R, k, sZ, bits, nullptr, blockIdx.x * WG + threadIdx.x, G * WG);
+    mark_body<false>(R, k, k, sZ, bits, nullptr, blockIdx.x * WG + threadIdx.x, G * WG);
   grid_barrier(ctrl, base + G);
   // every workgroup has read `base` before arriving: publish the next launch's base
   if (blockIdx.x == 0 && threadIdx.x == 0) st_relaxed((uint64_t*)&ctrl[kCtrlBarEpoch], base + 2ull * G);
   for (uint32_t s = blockIdx.x; s < ns; s += G) move_slice<T, WG, IPT, true>(A, R, Rd, k, sZ, L, s, bits, counts, msh);
   grid_barrier(ctrl, base + 2ull * G);
-  pair_body<T, WG, true>(A, Rd, sZ, L, ns, counts, ctrl, stats, blockIdx.x, G, psh);
+  pair_body<T, WG, kMaxSlices, true>(A, Rd, sZ, L, ns, counts, ctrl, stats, blockIdx.x, G, psh);
+}
+
+// ---------------------------------------------------------------------------
+// ordered single launch: workgroups take their role from their id,
+// [0, nm) mark tiles of LM removals, [nm, nm+ns) move slices, the rest pair
+// shares.  Workgroups are dispatched in id order and each waits only on lower
+// ids, so the launch needs neither a cooperative launch nor co-residency.
+// Every workgroup does one item and exits, so the random stores of one slice
+// never hold up the loads of the next (a persistent workgroup's vmcnt would).
+// The last workgroup to leave resets the three counters.
+// ---------------------------------------------------------------------------
+enum OrderedWord : int { kOrdMark = 0, kOrdMove = 32, kOrdExit = 64 };
+constexpr uint32_t kOrdMarkTile = 4096;
+constexpr int kOrdCtlOffset = 128;  // words into the context's queue block
+
+template <typename T, int WG, int IPT>
+__global__ __launch_bounds__(WG) void rz_ordered(T* __restrict__ A, const int32_t* __restrict__ R,
+                                                  int32_t* __restrict__ Rd, uint32_t k, uint32_t sZ, uint32_t L,
+                                                  uint32_t ns, uint32_t nm, uint32_t* __restrict__ bits,
+                                                  uint64_t* __restrict__ counts, uint32_t* __restrict__ ctrl,
+                                                  uint32_t* __restrict__ q, int stats) {
+  __shared__ MoveShared<WG, IPT> msh;
+  __shared__ uint32_t s_last;
+  extern __shared__ uint64_t dyn[];  // pairing prefix: ns + 1 entries, then WG/64 + 1 scan cells
+  const uint32_t b = blockIdx.x, tid = threadIdx.x;
+  if (b < nm) {
+    const uint32_t lo = b * kOrdMarkTile, hi = min(lo + kOrdMarkTile, k);
+    if (stats)
+      mark_body<true>(R + lo, hi - lo, k, sZ, bits, (uint64_t*)&ctrl[kCtrlStatTail], tid, WG);
+    else
+      mark_body<false>(R + lo, hi - lo, k, sZ, bits, nullptr, tid, WG);
+    signal_done(&q[kOrdMark]);
+  } else if (b < nm + ns) {
+    wait_count(&q[kOrdMark], nm, ctrl);
+    move_slice<T, WG, IPT, true>(A, R, Rd, k, sZ, L, b - nm, bits, counts, msh, &q[kOrdMove]);
+  } else {
+    wait_count(&q[kOrdMove], ns, ctrl);
+    pair_body_lds<T, WG, true>(A, Rd, sZ, L, ns, counts, ctrl, stats, b - nm - ns, gridDim.x - nm - ns, dyn,
+                               dyn + ns + 1);
+  }
+  if (tid == 0) {
+    wait_all_memory();
+    s_last = add_relaxed(&q[kOrdExit], 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (s_last && tid == 0) {
+    st_relaxed(&q[kOrdMark], 0u);
+    st_relaxed(&q[kOrdMove], 0u);
+    st_relaxed(&q[kOrdExit], 0u);
+  }
 }
 
 // ---------------------------------------------------------------------------
 // launch
 // ---------------------------------------------------------------------------
+template <typename T, int WG, int IPT>
+static rz_status ordered_launch(rz_ctx* ctx, T* A, int32_t* R, int32_t* Rd, uint32_t k, uint32_t sZ, uint32_t L,
+                                uint32_t ns, uint32_t* bits, uint64_t* counts, uint32_t* ctrl, int stats,
+                                hipStream_t s) {
+  const uint32_t nm = (k + kOrdMarkTile - 1) / kOrdMarkTile;
+  uint32_t np = ns / 8;
+  if (np < 1) np = 1;
+  if (np > (uint32_t)ctx->cu_count) np = (uint32_t)ctx->cu_count;
+  const size_t lds = ((size_t)ns + 1 + WG / kWave + 1) * sizeof(uint64_t);
+  rz_status st;
+  if ((st = ensure(ctx, ctx->queue, 4096, s)) != RZ_OK) return st;
+  uint32_t* q = (uint32_t*)ctx->queue.ptr + kOrdCtlOffset;
+  hipLaunchKernelGGL((rz_ordered<T, WG, IPT>), dim3(nm + ns + np), dim3(WG), lds, s, A, R, Rd, k, sZ, L, ns, nm,
+                     bits, counts, ctrl, q, stats);
+  RZ_HIP(hipGetLastError());
+  return RZ_OK;
+}
+
+template <typename T>
+static rz_status dispatch_ordered(int wg, int ipt, rz_ctx* ctx, T* A, int32_t* R, int32_t* Rd, uint32_t k,
+                                  uint32_t sZ, uint32_t L, uint32_t ns, uint32_t* bits, uint64_t* counts,
+                                  uint32_t* ctrl, int stats, hipStream_t s) {
+#define RZ_ORD_CASE(W, I) \
+  if (wg == W && ipt == I) return ordered_launch<T, W, I>(ctx, A, R, Rd, k, sZ, L, ns, bits, counts, ctrl, stats, s);
+  RZ_ORD_CASE(256, 2) RZ_ORD_CASE(256, 4) RZ_ORD_CASE(256, 8) RZ_ORD_CASE(256, 16)
+  RZ_ORD_CASE(512, 2) RZ_ORD_CASE(512, 4) RZ_ORD_CASE(512, 8) RZ_ORD_CASE(512, 16)
+#undef RZ_ORD_CASE
+  return RZ_EINVAL;
+}
 template <typename T, int WG, int IPT>
 static void launch_move(dim3 grid, hipStream_t s, T* A, const int32_t* R, int32_t* Rd, uint32_t k, uint32_t sZ,
                         uint32_t L, uint32_t* bits, uint64_t* counts) {
@@ -336,11 +201,10 @@ static bool dispatch_move(int wg, int ipt, dim3 grid, hipStream_t s, T* A, const
   return false;
 }
 
-// Slice length: a multiple of one step (wg * items), with at most max_slices slices.
-static void slice_shape(int64_t k, const rz_tile_config& cfg, uint32_t* L, uint32_t* ns) {
+void slice_shape(int64_t k, const rz_tile_config& cfg, int max_slices_cap, uint32_t* L, uint32_t* ns) {
   const int64_t step = (int64_t)cfg.wg_size * cfg.items;
   int64_t cap = cfg.max_slices < 1 ? 1 : cfg.max_slices;
-  if (cap > kMaxSlices) cap = kMaxSlices;
+  if (cap > max_slices_cap) cap = max_slices_cap;
   const int64_t steps = (k + step * cap - 1) / (step * cap);
   *L = (uint32_t)(steps * step);
   *ns = (uint32_t)((k + *L - 1) / *L);
@@ -379,7 +243,7 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
                             const rz_tile_config& cfg, bool coop, hipStream_t s) {
   const uint32_t sZ = (uint32_t)(n - k);
   uint32_t L, ns;
-  slice_shape(k, cfg, &L, &ns);
+  slice_shape(k, cfg, kMaxSlices, &L, &ns);
   rz_status st;
   if ((st = ensure(ctx, ctx->tail_bits, (size_t)((k + 63) / 64) * 8, s)) != RZ_OK) return st;
   if ((st = ensure(ctx, ctx->counts, (size_t)ns * 8, s)) != RZ_OK) return st;
@@ -394,9 +258,14 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
   const int stats = (flags & RZ_STATS) ? 1 : 0;
   if (stats && (st = fill_u32(ctrl + kCtrlStatTail, 0u, 2, s)) != RZ_OK) return st;
   if ((st = prof_mark(ctx, flags, 0, s)) != RZ_OK) return st;
-  if (coop) {
+  if (coop || cfg.variant == 3) {
     if ((st = prof_mark(ctx, flags, 1, s)) != RZ_OK) return st;
-    st = dispatch_coop<T>(cfg.wg_size, cfg.items, ctx, A, R, Rd, (uint32_t)k, sZ, L, ns, bits, counts, ctrl, stats, s);
+    if (coop)
+      st = dispatch_coop<T>(cfg.wg_size, cfg.items, ctx, A, R, Rd, (uint32_t)k, sZ, L, ns, bits, counts, ctrl, stats,
+                            s);
+    else
+      st = dispatch_ordered<T>(cfg.wg_size, cfg.items, ctx, A, R, Rd, (uint32_t)k, sZ, L, ns, bits, counts, ctrl,
+                               stats, s);
     if (st != RZ_OK) return st;
     if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;
     return prof_mark(ctx, flags, 3, s);
@@ -422,6 +291,7 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
 
 rz_status launch_plain(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, int32_t* R, int64_t k, uint32_t flags,
                        const rz_tile_config& cfg, hipStream_t s) {
+  if (cfg.variant == 2) return launch_queue_single(ctx, data, n, esz, R, k, flags, cfg, s);
   const bool coop = cfg.variant == 1 || (flags & RZ_COOPERATIVE);
   switch (esz) {
     case 4: return plain_impl<uint32_t>(ctx, (uint32_t*)data, n, R, k, flags, cfg, coop, s);

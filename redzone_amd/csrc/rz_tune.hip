@@ -1,0 +1,326 @@
+// rz_tune.hip -- autotuner of launch shapes.
+//
+// For one (element size, n, k) the tuner builds a synthetic list and removal
+// list on the device (rz_gen_*), times every candidate launch shape with
+// events on a private stream (restoring the inputs and, for cold timing,
+// reading a 1 GiB scrub buffer before each call), and installs the fastest in
+// the context's table under (element size, floor(log2 k)).  The batch tuner
+// does the same for the persistent queue on a synthetic batch of lists.  The
+// table is saved and loaded as plain text lines.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <random>
+#include <vector>
+
+#include "rz_ctx.hpp"
+#include "rz_device.hpp"
+
+namespace rz {
+
+__global__ __launch_bounds__(256) void rz_scrub_kernel(const uint4* __restrict__ p, uint64_t n16,
+                                                        uint32_t* __restrict__ sink) {
+  uint32_t acc = 0;
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n16; i += gridDim.x * 256ull) {
+    const uint4 v = p[i];
+    acc ^= v.x ^ v.y ^ v.z ^ v.w;
+  }
+  acc = wave_sum(acc);
+  if (acc == 0x9e3779b9u && lane_id() == 0) sink[0] = acc;  // keeps the loads alive; practically never true
+}
+
+struct Scratch {
+  std::vector<void*> ptrs;
+  ~Scratch() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  void* alloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) return nullptr;
+    ptrs.push_back(p);
+    return p;
+  }
+};
+
+static float median(std::vector<float> v) {
+  std::sort(v.begin(), v.end());
+  return v.empty() ? 0.f : v[v.size() / 2];
+}
+
+static const rz_tile_config kCandidates[] = {
+    // wg, items, max_slices, variant, slice_steps, chunk
+    {256, 4, 2048, 0, 0, 1},  {256, 8, 4096, 0, 0, 1},  {512, 8, 4096, 0, 0, 1},  {512, 16, 4096, 0, 0, 1},
+    {256, 4, 512, 0, 0, 1},   {512, 8, 1024, 0, 0, 1},  {256, 8, 4096, 3, 0, 1},  {512, 8, 4096, 3, 0, 1},
+    {256, 4, 4096, 3, 0, 1},  {256, 8, 0, 2, 1, 1},     {512, 8, 0, 2, 1, 1},     {256, 4, 0, 2, 2, 1},
+};
+
+static const rz_tile_config kBatchCandidates[] = {
+    {256, 4, 0, 2, 1, 1}, {256, 4, 0, 2, 4, 1}, {256, 8, 0, 2, 1, 1}, {256, 8, 0, 2, 2, 1},
+    {256, 8, 0, 2, 4, 1}, {512, 8, 0, 2, 1, 1}, {512, 8, 0, 2, 2, 1}, {512, 4, 0, 2, 2, 1},
+    {256, 8, 0, 2, 2, 4}, {512, 8, 0, 2, 1, 2},
+};
+
+}  // namespace rz
+
+using namespace rz;
+
+extern "C" {
+
+rz_status rz_tune_candidates(int32_t batch, rz_tile_config* out, int32_t max, int32_t* count) {
+  const rz_tile_config* src = batch ? kBatchCandidates : kCandidates;
+  const int32_t n = batch ? (int32_t)(sizeof(kBatchCandidates) / sizeof(kBatchCandidates[0]))
+                          : (int32_t)(sizeof(kCandidates) / sizeof(kCandidates[0]));
+  if (count) *count = n;
+  for (int32_t i = 0; out && i < n && i < max; ++i) out[i] = src[i];
+  return RZ_OK;
+}
+
+rz_status rz_table_get(rz_ctx* ctx, uint32_t elem_bytes, int32_t log2k, rz_tile_config* out) {
+  if (!ctx || !out || esz_index(elem_bytes) < 0 || log2k < 0 || log2k > 32) return RZ_EINVAL;
+  *out = ctx->table[esz_index(elem_bytes)][log2k];
+  return RZ_OK;
+}
+
+rz_status rz_table_set(rz_ctx* ctx, uint32_t elem_bytes, int32_t log2k, const rz_tile_config* cfg) {
+  if (!ctx || !cfg || esz_index(elem_bytes) < 0 || log2k < 0 || log2k > 32) return RZ_EINVAL;
+  ctx->table[esz_index(elem_bytes)][log2k] = *cfg;
+  return RZ_OK;
+}
+
+rz_status rz_batch_config_get(rz_ctx* ctx, rz_tile_config* out) {
+  if (!ctx || !out) return RZ_EINVAL;
+  *out = ctx->batch_cfg;
+  return RZ_OK;
+}
+
+rz_status rz_batch_config_set(rz_ctx* ctx, const rz_tile_config* cfg) {
+  if (!ctx || !cfg || cfg->variant != 2) return RZ_EINVAL;
+  ctx->batch_cfg = *cfg;
+  return RZ_OK;
+}
+
+// Text format, one line per entry:
+//   plain <elem_bytes> <log2k> <wg> <items> <max_slices> <variant> <slice_steps> <chunk>
+//   batch <wg> <items> <max_slices> <variant> <slice_steps> <chunk>
+rz_status rz_table_save(rz_ctx* ctx, const char* path) {
+  if (!ctx || !path) return RZ_EINVAL;
+  FILE* f = fopen(path, "w");
+  if (!f) return RZ_EINVAL;
+  fprintf(f, "# redzone_amd launch-shape table (%s, %d CUs)\n", ctx->arch, ctx->cu_count);
+  for (int e = 0; e < 3; ++e)
+    for (int b = 0; b <= 32; ++b) {
+      const rz_tile_config& c = ctx->table[e][b];
+      fprintf(f, "plain %d %d %d %d %d %d %d %d\n", 4 << e, b, c.wg_size, c.items, c.max_slices, c.variant,
+              c.slice_steps, c.chunk);
+    }
+  const rz_tile_config& c = ctx->batch_cfg;
+  fprintf(f, "batch %d %d %d %d %d %d\n", c.wg_size, c.items, c.max_slices, c.variant, c.slice_steps, c.chunk);
+  fclose(f);
+  return RZ_OK;
+}
+
+rz_status rz_table_load(rz_ctx* ctx, const char* path) {
+  if (!ctx || !path) return RZ_EINVAL;
+  FILE* f = fopen(path, "r");
+  if (!f) return RZ_EINVAL;
+  char line[256];
+  rz_status st = RZ_OK;
+  while (fgets(line, sizeof(line), f)) {
+    if (line[0] == '#' || line[0] == '\n') continue;
+    rz_tile_config c;
+    int e = 0, b = 0;
+    if (sscanf(line, "plain %d %d %d %d %d %d %d %d", &e, &b, &c.wg_size, &c.items, &c.max_slices, &c.variant,
+               &c.slice_steps, &c.chunk) == 8) {
+      if (esz_index((uint32_t)e) < 0 || b < 0 || b > 32) { st = RZ_EINVAL; break; }
+      ctx->table[esz_index((uint32_t)e)][b] = c;
+    } else if (sscanf(line, "batch %d %d %d %d %d %d", &c.wg_size, &c.items, &c.max_slices, &c.variant,
+                      &c.slice_steps, &c.chunk) == 6) {
+      ctx->batch_cfg = c;
+    } else {
+      st = RZ_EINVAL;
+      break;
+    }
+  }
+  fclose(f);
+  return st;
+}
+
+// Times each candidate (median of `reps` calls after one warm-up call) on a
+// synthetic list; all_ms (ncand entries, may be null) receives the medians.
+// The fastest is installed in the table.  Allocates its own inputs.
+rz_status rz_tune(rz_ctx* ctx, uint32_t elem_bytes, int64_t n, int64_t k, const rz_tile_config* cands,
+                  int32_t ncand, int32_t reps, int32_t cold, rz_tile_config* best, float* best_ms, float* all_ms) {
+  if (!ctx || esz_index(elem_bytes) < 0 || n < 1 || k < 1 || k >= n || n >= (1ll << 31) || reps < 1) return RZ_EINVAL;
+  if (!cands) {
+    cands = kCandidates;
+    ncand = (int32_t)(sizeof(kCandidates) / sizeof(kCandidates[0]));
+  }
+  if (ncand < 1) return RZ_EINVAL;
+  int dev_prev = -1;
+  (void)hipGetDevice(&dev_prev);
+  if (dev_prev != ctx->device) (void)hipSetDevice(ctx->device);
+  Scratch sc;
+  const size_t bytes = (size_t)n * elem_bytes;
+  void* data = sc.alloc(bytes);
+  void* pristine = sc.alloc(bytes);
+  int32_t* R = (int32_t*)sc.alloc((size_t)k * 4);
+  int32_t* R0 = (int32_t*)sc.alloc((size_t)k * 4);
+  const uint64_t scrub_bytes = cold ? (1ull << 30) : 0;
+  uint4* scrub = cold ? (uint4*)sc.alloc(scrub_bytes) : nullptr;
+  uint32_t* sink = (uint32_t*)sc.alloc(256);
+  if (!data || !pristine || !R || !R0 || (cold && !scrub) || !sink) return RZ_ENOMEM;
+  hipStream_t s;
+  RZ_HIP(hipStreamCreate(&s));
+  hipEvent_t e0, e1;
+  RZ_HIP(hipEventCreate(&e0));
+  RZ_HIP(hipEventCreate(&e1));
+  rz_status st = rz_gen_payload(ctx, pristine, n, elem_bytes, 0x5eed0001ull, s);
+  if (st == RZ_OK) st = rz_gen_uniform_removals(ctx, R0, n, k, 0x5eed0002ull, s);
+  if (cold && st == RZ_OK) (void)hipMemsetAsync(scrub, 0x5a, scrub_bytes, s);
+  int best_i = -1;
+  float best_t = 0.f;
+  for (int32_t c = 0; c < ncand && st == RZ_OK; ++c) {
+    std::vector<float> ts;
+    for (int32_t r = 0; r <= reps && st == RZ_OK; ++r) {
+      if (hipMemcpyAsync(data, pristine, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+          hipMemcpyAsync(R, R0, (size_t)k * 4, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+        st = RZ_EHIP;
+        break;
+      }
+      if (cold)
+        hipLaunchKernelGGL(rz_scrub_kernel, dim3((uint32_t)ctx->cu_count * 8), dim3(256), 0, s, scrub,
+                           scrub_bytes / 16, sink);
+      (void)hipEventRecord(e0, s);
+      st = rz_remove_config(ctx, data, n, elem_bytes, R, k, 0, &cands[c], s);
+      (void)hipEventRecord(e1, s);
+      if (hipEventSynchronize(e1) != hipSuccess) st = RZ_EHIP;
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (r > 0) ts.push_back(ms);
+    }
+    if (st != RZ_OK) {
+      // a shape the device refuses (e.g. a cooperative grid too large) is skipped
+      if (st == RZ_ECOOP || st == RZ_EINVAL) {
+        st = RZ_OK;
+        if (all_ms) all_ms[c] = -1.f;
+        continue;
+      }
+      break;
+    }
+    const float m = median(ts);
+    if (all_ms) all_ms[c] = m;
+    if (best_i < 0 || m < best_t) {
+      best_i = c;
+      best_t = m;
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  if (dev_prev >= 0 && dev_prev != ctx->device) (void)hipSetDevice(dev_prev);
+  if (st != RZ_OK) return st;
+  if (best_i < 0) return RZ_EINVAL;
+  ctx->table[esz_index(elem_bytes)][log2_floor(k)] = cands[best_i];
+  if (best) *best = cands[best_i];
+  if (best_ms) *best_ms = best_t;
+  return RZ_OK;
+}
+
+// Batch tuner: nlists lists with n log-uniform in [n_min, n_max] and
+// k = floor(r n), r uniform in [0, r_max] (a config-4-like batch), all of
+// elem_bytes; times create-once / launch-many of each candidate.
+rz_status rz_tune_batch(rz_ctx* ctx, uint32_t elem_bytes, int32_t nlists, int64_t n_min, int64_t n_max,
+                        double r_max, int32_t reps, const rz_tile_config* cands, int32_t ncand,
+                        rz_tile_config* best, float* best_ms, float* all_ms) {
+  if (!ctx || esz_index(elem_bytes) < 0 || nlists < 1 || n_min < 1 || n_max < n_min || n_max >= (1ll << 31) ||
+      r_max < 0 || r_max > 1 || reps < 1)
+    return RZ_EINVAL;
+  if (!cands) {
+    cands = kBatchCandidates;
+    ncand = (int32_t)(sizeof(kBatchCandidates) / sizeof(kBatchCandidates[0]));
+  }
+  int dev_prev = -1;
+  (void)hipGetDevice(&dev_prev);
+  if (dev_prev != ctx->device) (void)hipSetDevice(ctx->device);
+  std::mt19937_64 rng(0x7a11ull);
+  std::uniform_real_distribution<double> u01(0.0, 1.0);
+  std::vector<int64_t> ns(nlists), ks(nlists), off(nlists + 1, 0), roff(nlists + 1, 0);
+  const double l0 = log((double)n_min), l1 = log((double)n_max);
+  for (int32_t i = 0; i < nlists; ++i) {
+    ns[i] = (int64_t)exp(l0 + (l1 - l0) * u01(rng));
+    if (ns[i] < 1) ns[i] = 1;
+    ks[i] = (int64_t)(u01(rng) * r_max * (double)ns[i]);
+    off[i + 1] = off[i] + ns[i];
+    roff[i + 1] = roff[i] + ks[i];
+  }
+  Scratch sc;
+  const size_t bytes = (size_t)off[nlists] * elem_bytes;
+  char* data = (char*)sc.alloc(bytes);
+  char* pristine = (char*)sc.alloc(bytes);
+  int32_t* R = (int32_t*)sc.alloc((size_t)roff[nlists] * 4 + 4);
+  int32_t* R0 = (int32_t*)sc.alloc((size_t)roff[nlists] * 4 + 4);
+  if (!data || !pristine || !R || !R0) return RZ_ENOMEM;
+  hipStream_t s;
+  RZ_HIP(hipStreamCreate(&s));
+  rz_status st = RZ_OK;
+  std::vector<rz_list_desc> descs(nlists);
+  for (int32_t i = 0; i < nlists && st == RZ_OK; ++i) {
+    st = rz_gen_payload(ctx, pristine + (size_t)off[i] * elem_bytes, ns[i], elem_bytes, 100 + i, s);
+    if (st == RZ_OK) st = rz_gen_uniform_removals(ctx, R0 + roff[i], ns[i], ks[i], 200 + i, s);
+    descs[i].data = data + (size_t)off[i] * elem_bytes;
+    descs[i].removals = R + roff[i];
+    descs[i].n = ns[i];
+    descs[i].k = ks[i];
+    descs[i].elem_bytes = elem_bytes;
+    descs[i].method = 0;
+  }
+  hipEvent_t e0, e1;
+  RZ_HIP(hipEventCreate(&e0));
+  RZ_HIP(hipEventCreate(&e1));
+  int best_i = -1;
+  float best_t = 0.f;
+  for (int32_t c = 0; c < ncand && st == RZ_OK; ++c) {
+    rz_batch* b = nullptr;
+    st = rz_batch_create(ctx, descs.data(), nlists, 0, &cands[c], &b);
+    if (st != RZ_OK) break;
+    std::vector<float> ts;
+    for (int32_t r = 0; r <= reps && st == RZ_OK; ++r) {
+      if (hipMemcpyAsync(data, pristine, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+          hipMemcpyAsync(R, R0, (size_t)roff[nlists] * 4, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+        st = RZ_EHIP;
+        break;
+      }
+      (void)hipEventRecord(e0, s);
+      st = rz_batch_launch(b, s);
+      (void)hipEventRecord(e1, s);
+      if (hipEventSynchronize(e1) != hipSuccess) st = RZ_EHIP;
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (r > 0) ts.push_back(ms);
+    }
+    rz_batch_destroy(b);
+    if (st != RZ_OK) break;
+    const float m = median(ts);
+    if (all_ms) all_ms[c] = m;
+    if (best_i < 0 || m < best_t) {
+      best_i = c;
+      best_t = m;
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  if (dev_prev >= 0 && dev_prev != ctx->device) (void)hipSetDevice(dev_prev);
+  if (st != RZ_OK) return st;
+  if (best_i < 0) return RZ_EINVAL;
+  ctx->batch_cfg = cands[best_i];
+  if (best) *best = cands[best_i];
+  if (best_ms) *best_ms = best_t;
+  return RZ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,58 @@
+#!/usr/bin/env python3
+"""Autotunes the launch shapes on this device and writes the table that
+Context loads by default (redzone_amd/tuned_gfx950.txt).
+
+    python -m redzone_amd.tune [--out PATH] [--reps 5] [--quick]
+
+For every element size (4, 8, 16 bytes) and every k bucket 2^b (k < n/2) the
+native tuner (rz_tune) times the built-in candidate shapes on a synthetic
+list of 1 GiB with cold caches and keeps the fastest; then the batch tuner
+(rz_tune_batch) picks the queue shape on a 4096-list batch shaped like the
+batched benchmark.  Every line of the result is printed as JSON.
+"""
+import argparse
+import json
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def main():
+    import redzone_amd as rz
+    from redzone_amd import _native
+
+    p = argparse.ArgumentParser()
+    p.add_argument("--out", default=_native.TUNED_TABLE)
+    p.add_argument("--reps", type=int, default=5)
+    args = p.parse_args()
+    ctx = rz.Context(0, load_tuned=False)
+    print(json.dumps({"device": ctx.device_info()}), flush=True)
+    lo_b = 8
+    for esz in (4, 8, 16):
+        n = (1 << 30) // esz
+        hi_b = lo_b
+        for b in range(lo_b, 32):
+            k = 1 << b
+            if k * 2 > n:
+                break
+            best, ms, results = ctx.tune(esz, n, k, reps=args.reps, cold=True)
+            hi_b = b
+            print(json.dumps({"elem_bytes": esz, "n": n, "log2k": b, "best": best.as_tuple(), "best_ms": round(ms, 5),
+                              "all": [[c.as_tuple(), round(t, 5)] for c, t in results]}), flush=True)
+        # buckets outside the tuned range take the nearest tuned shape
+        for bb in range(hi_b + 1, 33):
+            ctx.set_table(esz, bb, ctx.table(esz, hi_b))
+        for bb in range(0, lo_b):
+            ctx.set_table(esz, bb, ctx.table(esz, lo_b))
+    best, ms, results = ctx.tune_batch(4, 4096, 1 << 8, 1 << 20, 0.3, reps=max(2, args.reps // 2))
+    print(json.dumps({"batch": best.as_tuple(), "best_ms": round(ms, 4),
+                      "all": [[c.as_tuple(), round(t, 4)] for c, t in results]}), flush=True)
+    ctx.save_table(args.out)
+    print(json.dumps({"saved": args.out}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

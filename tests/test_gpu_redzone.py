@@ -63,7 +63,7 @@ def test_patterns_and_sizes(ctx, oracle, esz, pattern, frac):
     gpu_remove_parity(ctx, oracle, A, R)
 
 
-CONFIGS = [rz.TileConfig(wg, it, ms, 0) for wg in (256, 512) for it in (2, 4, 8, 16) for ms in (1, 3, 64, 4096)]
+CONFIGS = [rz.TileConfig(wg, it, ms, 0, 0, 0) for wg in (256, 512) for it in (2, 4, 8, 16) for ms in (1, 3, 64, 4096)]
 
 
 @pytest.mark.parametrize("esz", [4, 16])
@@ -88,9 +88,40 @@ def test_cooperative_variant(ctx, oracle, esz):
         A = _payload(esz, n, seed=31)
         R = W.tail_heavy_removals(n, k, seed=32)
         for wg, it in ((256, 4), (512, 8)):
-            cfg = rz.TileConfig(wg, it, ms, 1)
+            cfg = rz.TileConfig(wg, it, ms, 1, 0, 0)
             announce("coop", esz, n, k, cfg)
             gpu_remove_parity(ctx, oracle, A, R, config=cfg)
+
+
+@pytest.mark.parametrize("esz", [4, 8, 16])
+def test_ordered_variant(ctx, oracle, esz):
+    """The ordered single launch (roles by workgroup id, in-kernel counters) gives the same array."""
+    for n, frac, ms, pattern in [((1 << 18) + 5, 0.01, 4096, "uniform"), ((1 << 18) + 5, 0.4, 64, "tail_heavy"),
+                                 (1 << 16, 0.9, 7, "clustered"), (1 << 20, 0.05, 4096, "tail_heavy"),
+                                 (3000, 0.5, 4096, "uniform"), (70000, 0.001, 1, "uniform")]:
+        k = int(n * frac)
+        A = _payload(esz, n, seed=33)
+        R = W.removals(pattern, n, k, seed=34)
+        want = expected_canonical(oracle, A.numpy(), R.numpy())
+        for wg, it in ((256, 4), (256, 16), (512, 8)):
+            cfg = rz.TileConfig(wg, it, ms, 3, 0, 0)
+            announce("ordered", esz, n, k, pattern, cfg)
+            gpu_remove_parity(ctx, oracle, A, R, config=cfg, want=want)
+
+
+def test_ordered_variant_stats_and_preserve(ctx, oracle):
+    n = 1 << 18
+    k = n // 10
+    A = W.index_payload(n)
+    R = W.tail_heavy_removals(n, k, seed=35)
+    cfg = rz.TileConfig(256, 8, 4096, 3, 0, 0)
+    announce("ordered stats")
+    _, dR = gpu_remove_parity(ctx, oracle, A, R, config=cfg, flags=rz.RZ_STATS | rz.RZ_PRESERVE_REMOVALS)
+    s = ctx.last_stats()
+    Rn = R.numpy()
+    assert s.tail_removals == int((Rn >= n - k).sum())
+    assert s.error_flags == 0
+    assert torch.equal(dR.cpu(), R)
 
 
 def test_ragged_small(ctx, oracle):
