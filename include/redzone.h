@@ -200,6 +200,37 @@ rz_status rz_batch_config_set(rz_ctx* ctx, const rz_tile_config* cfg);
 rz_status rz_table_save(rz_ctx* ctx, const char* path);
 rz_status rz_table_load(rz_ctx* ctx, const char* path);
 
+/* ---- learned dispatch ---------------------------------------------------- */
+/* No counterpart in the reference, which states the fixed rule "Redzone is
+ * faster than O(n) when k < n/2" (README.md:6; paper p.14).  A small FP32 MLP
+ * predicts the log2 time of each method from features of a list; the
+ * dispatcher runs the method with the smallest prediction.  Layers F -> H ->
+ * H -> H -> C with ReLU; parameters are row-major W[out][in] then b[out] per
+ * layer, in layer order (RZ_MLP_PARAMS floats). */
+#define RZ_MLP_FEATURES 12
+#define RZ_MLP_HIDDEN 32
+#define RZ_MLP_OUTPUTS 4
+#define RZ_MLP_PARAMS 2660
+/* Features of each list (device array of descriptors) into d_features
+ * [nlists][RZ_MLP_FEATURES]: log2 n / 32, log2 k / 32, k/n, log2(k/n) / 20,
+ * elem_bytes / 16, and from a 256-entry strided sample of the removal list:
+ * fraction in the red zone, mean and std of position / n, fraction of adjacent
+ * entries that are consecutive / within 64, fraction ascending; then 1. */
+rz_status rz_features(rz_ctx* ctx, const rz_list_desc* d_lists, int32_t nlists, float* d_features, void* stream);
+/* Forward pass: d_y [batch][RZ_MLP_OUTPUTS] (may be null) and the argmin per
+ * row in d_choice (may be null). */
+rz_status rz_mlp_forward(rz_ctx* ctx, const float* d_params, const float* d_x, int32_t batch, float* d_y,
+                         int32_t* d_choice, void* stream);
+/* Gradient of the masked mean squared error sum m (y - t)^2 / mask_sum over
+ * the batch into d_grad [RZ_MLP_PARAMS] and the loss into d_loss (may be
+ * null).  d_work holds batch * (RZ_MLP_PARAMS + 1) floats.  Deterministic. */
+rz_status rz_mlp_grad(rz_ctx* ctx, const float* d_params, const float* d_x, const float* d_target,
+                      const float* d_mask, int32_t batch, float mask_sum, float* d_grad, float* d_loss,
+                      float* d_work, void* stream);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with step >= 1. */
+rz_status rz_mlp_adam(rz_ctx* ctx, float* d_params, const float* d_grad, float* d_m, float* d_v, int32_t step,
+                      float lr, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,6 +114,10 @@ SIGNATURES = [
     ("rz_batch_config_set", _i32, [_vp, ctypes.POINTER(TileConfig)]),
     ("rz_table_save", _i32, [_vp, ctypes.c_char_p]),
     ("rz_table_load", _i32, [_vp, ctypes.c_char_p]),
+    ("rz_features", _i32, [_vp, _vp, _i32, _vp, _vp]),
+    ("rz_mlp_forward", _i32, [_vp, _vp, _vp, _i32, _vp, _vp, _vp]),
+    ("rz_mlp_grad", _i32, [_vp, _vp, _vp, _vp, _vp, _i32, ctypes.c_float, _vp, _vp, _vp, _vp]),
+    ("rz_mlp_adam", _i32, [_vp, _vp, _vp, _vp, _vp, _i32, ctypes.c_float, _vp]),
 ]
 
 TUNED_TABLE = os.path.join(_HERE, "tuned_gfx950.txt")

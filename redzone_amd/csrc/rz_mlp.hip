@@ -1,0 +1,274 @@
+// rz_mlp.hip -- learned dispatch: list features and a small FP32 MLP with
+// hand-written inference and training kernels.
+//
+// The MLP maps RZ_MLP_FEATURES features of a (list, removal list) pair to the
+// predicted log2 time (us) of each of RZ_MLP_OUTPUTS methods; the dispatcher
+// runs the method with the smallest prediction.  Layers: F -> 32 -> 32 -> 32
+// -> C, ReLU between them, parameters row-major W[out][in] followed by b[out]
+// for each layer, in order.  Training minimises the masked mean squared error
+// on measured log times with Adam.  Per-sample gradients are written out and
+// summed in a fixed order, so a gradient is the same bits on every run.
+#include "rz_ctx.hpp"
+#include "rz_device.hpp"
+
+namespace rz {
+
+constexpr int F = RZ_MLP_FEATURES, H = RZ_MLP_HIDDEN, C = RZ_MLP_OUTPUTS;
+constexpr int OW1 = 0, OB1 = OW1 + H * F, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3 = OB2 + H, OB3 = OW3 + H * H,
+              OW4 = OB3 + H, OB4 = OW4 + C * H, NP = OB4 + C;
+static_assert(NP == RZ_MLP_PARAMS, "parameter count");
+
+// ---------------------------------------------------------------------------
+// features: one wave per list, a 256-entry strided sample of the removal list
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void rz_features_kernel(const rz_list_desc* __restrict__ lists, int32_t nlists,
+                                                          float* __restrict__ out) {
+  const int li = blockIdx.x;
+  if (li >= nlists) return;
+  const rz_list_desc d = lists[li];
+  const int lane = threadIdx.x;
+  __shared__ float s_x[256];
+  const double n = (double)(d.n > 0 ? d.n : 1);
+  const int64_t k = d.k;
+  const int64_t sz = d.n - d.k;
+  const int m = k < 256 ? (int)k : 256;  // sample size
+  float sum = 0.f, sum2 = 0.f, tail = 0.f;
+  for (int j = lane; j < m; j += 64) {
+    const int64_t i = (int64_t)j * k / m;
+    const int64_t r = d.removals[i];
+    const float x = (float)((double)r / n);
+    s_x[j] = x;
+    sum += x;
+    sum2 += x * x;
+    tail += (r >= sz) ? 1.f : 0.f;
+  }
+  // adjacent pairs (R[i], R[i+1]) at 128 strided positions: runs of the removal list
+  float consec = 0.f, near = 0.f;
+  const int mp = k < 2 ? 0 : (k - 1 < 128 ? (int)(k - 1) : 128);
+  for (int j = lane; j < mp; j += 64) {
+    const int64_t i = (int64_t)j * (k - 1) / mp;
+    const int64_t a = d.removals[i], b = d.removals[i + 1];
+    consec += (b == a + 1) ? 1.f : 0.f;
+    near += (b - a < 64 && a - b < 64) ? 1.f : 0.f;
+  }
+  __syncthreads();
+  float up = 0.f;
+  for (int j = lane; j + 1 < m; j += 64) up += s_x[j + 1] > s_x[j] ? 1.f : 0.f;
+  sum = wave_sum(sum);
+  sum2 = wave_sum(sum2);
+  tail = wave_sum(tail);
+  consec = wave_sum(consec);
+  near = wave_sum(near);
+  up = wave_sum(up);
+  if (lane == 0) {
+    float* o = out + (size_t)li * F;
+    const float fm = m > 0 ? (float)m : 1.f;
+    const float mean = sum / fm;
+    const float var = fmaxf(sum2 / fm - mean * mean, 0.f);
+    const double frac = (double)k / n;
+    o[0] = (float)(log2(n) / 32.0);
+    o[1] = k > 0 ? (float)(log2((double)k) / 32.0) : 0.f;
+    o[2] = (float)frac;
+    o[3] = k > 0 ? (float)fmax(log2(frac) / 20.0, -1.0) : -1.f;
+    o[4] = (float)d.elem_bytes / 16.f;
+    o[5] = m > 0 ? tail / fm : 0.f;
+    o[6] = mean;
+    o[7] = sqrtf(var);
+    o[8] = mp > 0 ? consec / (float)mp : 0.f;
+    o[9] = mp > 0 ? near / (float)mp : 0.f;
+    o[10] = m > 1 ? up / (float)(m - 1) : 0.f;
+    o[11] = 1.f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// MLP
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void dense_relu(const float* __restrict__ W, const float* __restrict__ b,
+                                           const float* in, int nin, float* out_pre, float* out, int nout) {
+  for (int o = 0; o < nout; ++o) {
+    float acc = b[o];
+    for (int i = 0; i < nin; ++i) acc = fmaf(W[o * nin + i], in[i], acc);
+    out_pre[o] = acc;
+    out[o] = acc > 0.f ? acc : 0.f;
+  }
+}
+
+struct Acts {
+  float x[F];
+  float a1[H], h1[H], a2[H], h2[H], a3[H], h3[H];
+  float y[C];
+};
+
+__device__ __forceinline__ void forward(const float* __restrict__ P, const float* __restrict__ xin, Acts& t) {
+  for (int f = 0; f < F; ++f) t.x[f] = xin[f];
+  dense_relu(P + OW1, P + OB1, t.x, F, t.a1, t.h1, H);
+  dense_relu(P + OW2, P + OB2, t.h1, H, t.a2, t.h2, H);
+  dense_relu(P + OW3, P + OB3, t.h2, H, t.a3, t.h3, H);
+  for (int c = 0; c < C; ++c) {
+    float acc = P[OB4 + c];
+    for (int i = 0; i < H; ++i) acc = fmaf(P[OW4 + c * H + i], t.h3[i], acc);
+    t.y[c] = acc;
+  }
+}
+
+__global__ __launch_bounds__(64) void rz_mlp_forward_kernel(const float* __restrict__ params,
+                                                             const float* __restrict__ X, int32_t B,
+                                                             float* __restrict__ Y, int32_t* __restrict__ choice) {
+  __shared__ float P[NP];
+  for (int i = threadIdx.x; i < NP; i += blockDim.x) P[i] = params[i];
+  __syncthreads();
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  Acts t;
+  forward(P, X + (size_t)b * F, t);
+  int best = 0;
+  for (int c = 0; c < C; ++c) {
+    if (Y) Y[(size_t)b * C + c] = t.y[c];
+    if (t.y[c] < t.y[best]) best = c;
+  }
+  if (choice) choice[b] = best;
+}
+
+// Per-sample gradient of the masked MSE loss sum_b sum_c m_bc (y_bc - t_bc)^2 / sum(m)
+// into G[b][NP]; per-sample loss terms into Lb[b].
+__global__ __launch_bounds__(64) void rz_mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ X,
+                                                          const float* __restrict__ T, const float* __restrict__ M,
+                                                          int32_t B, float inv_msum, float* __restrict__ G,
+                                                          float* __restrict__ Lb) {
+  __shared__ float P[NP];
+  for (int i = threadIdx.x; i < NP; i += blockDim.x) P[i] = params[i];
+  __syncthreads();
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  Acts t;
+  forward(P, X + (size_t)b * F, t);
+  float* g = G + (size_t)b * NP;
+  float dy[C];
+  float loss = 0.f;
+  for (int c = 0; c < C; ++c) {
+    const float m = M[(size_t)b * C + c];
+    const float e = t.y[c] - T[(size_t)b * C + c];
+    loss += m * e * e;
+    dy[c] = 2.f * m * e * inv_msum;
+  }
+  Lb[b] = loss * inv_msum;
+  float d3[H], d2[H], d1[H];
+  // layer 4
+  for (int c = 0; c < C; ++c) {
+    g[OB4 + c] = dy[c];
+    for (int i = 0; i < H; ++i) g[OW4 + c * H + i] = dy[c] * t.h3[i];
+  }
+  for (int i = 0; i < H; ++i) {
+    float acc = 0.f;
+    for (int c = 0; c < C; ++c) acc = fmaf(P[OW4 + c * H + i], dy[c], acc);
+    d3[i] = t.a3[i] > 0.f ? acc : 0.f;
+  }
+  // layer 3
+  for (int o = 0; o < H; ++o) {
+    g[OB3 + o] = d3[o];
+    for (int i = 0; i < H; ++i) g[OW3 + o * H + i] = d3[o] * t.h2[i];
+  }
+  for (int i = 0; i < H; ++i) {
+    float acc = 0.f;
+    for (int o = 0; o < H; ++o) acc = fmaf(P[OW3 + o * H + i], d3[o], acc);
+    d2[i] = t.a2[i] > 0.f ? acc : 0.f;
+  }
+  // layer 2
+  for (int o = 0; o < H; ++o) {
+    g[OB2 + o] = d2[o];
+    for (int i = 0; i < H; ++i) g[OW2 + o * H + i] = d2[o] * t.h1[i];
+  }
+  for (int i = 0; i < H; ++i) {
+    float acc = 0.f;
+    for (int o = 0; o < H; ++o) acc = fmaf(P[OW2 + o * H + i], d2[o], acc);
+    d1[i] = t.a1[i] > 0.f ? acc : 0.f;
+  }
+  // layer 1
+  for (int o = 0; o < H; ++o) {
+    g[OB1 + o] = d1[o];
+    for (int f = 0; f < F; ++f) g[OW1 + o * F + f] = d1[o] * t.x[f];
+  }
+}
+
+// grad[p] = sum_b G[b][p] in ascending b (fixed order); loss = sum_b Lb[b].
+__global__ __launch_bounds__(256) void rz_mlp_reduce_kernel(const float* __restrict__ G, const float* __restrict__ Lb,
+                                                             int32_t B, float* __restrict__ grad,
+                                                             float* __restrict__ loss) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < NP) {
+    float acc = 0.f;
+    for (int b = 0; b < B; ++b) acc += G[(size_t)b * NP + p];
+    grad[p] = acc;
+  } else if (p == NP && loss) {
+    float acc = 0.f;
+    for (int b = 0; b < B; ++b) acc += Lb[b];
+    *loss = acc;
+  }
+}
+
+__global__ __launch_bounds__(256) void rz_mlp_adam_kernel(float* __restrict__ P, const float* __restrict__ g,
+                                                           float* __restrict__ m, float* __restrict__ v, float lr,
+                                                           float b1, float b2, float eps, float c1, float c2) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= NP) return;
+  const float gi = g[p];
+  const float mi = b1 * m[p] + (1.f - b1) * gi;
+  const float vi = b2 * v[p] + (1.f - b2) * gi * gi;
+  m[p] = mi;
+  v[p] = vi;
+  P[p] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+}  // namespace rz
+
+using namespace rz;
+
+extern "C" {
+
+rz_status rz_features(rz_ctx* ctx, const rz_list_desc* d_lists, int32_t nlists, float* d_features, void* stream) {
+  if (!ctx || nlists < 0 || (nlists > 0 && (!d_lists || !d_features))) return RZ_EINVAL;
+  if (nlists == 0) return RZ_OK;
+  hipLaunchKernelGGL(rz_features_kernel, dim3((uint32_t)nlists), dim3(64), 0, (hipStream_t)stream, d_lists, nlists,
+                     d_features);
+  RZ_HIP(hipGetLastError());
+  return RZ_OK;
+}
+
+rz_status rz_mlp_forward(rz_ctx* ctx, const float* d_params, const float* d_x, int32_t batch, float* d_y,
+                         int32_t* d_choice, void* stream) {
+  if (!ctx || !d_params || !d_x || batch < 0) return RZ_EINVAL;
+  if (batch == 0) return RZ_OK;
+  hipLaunchKernelGGL(rz_mlp_forward_kernel, dim3((uint32_t)((batch + 63) / 64)), dim3(64), 0, (hipStream_t)stream,
+                     d_params, d_x, batch, d_y, d_choice);
+  RZ_HIP(hipGetLastError());
+  return RZ_OK;
+}
+
+rz_status rz_mlp_grad(rz_ctx* ctx, const float* d_params, const float* d_x, const float* d_target,
+                      const float* d_mask, int32_t batch, float mask_sum, float* d_grad, float* d_loss,
+                      float* d_work, void* stream) {
+  if (!ctx || !d_params || !d_x || !d_target || !d_mask || !d_grad || !d_work || batch < 1 || !(mask_sum > 0.f))
+    return RZ_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  float* G = d_work;
+  float* Lb = d_work + (size_t)batch * NP;
+  hipLaunchKernelGGL(rz_mlp_grad_kernel, dim3((uint32_t)((batch + 63) / 64)), dim3(64), 0, s, d_params, d_x, d_target,
+                     d_mask, batch, 1.f / mask_sum, G, Lb);
+  hipLaunchKernelGGL(rz_mlp_reduce_kernel, dim3((NP + 1 + 255) / 256), dim3(256), 0, s, G, Lb, batch, d_grad, d_loss);
+  RZ_HIP(hipGetLastError());
+  return RZ_OK;
+}
+
+rz_status rz_mlp_adam(rz_ctx* ctx, float* d_params, const float* d_grad, float* d_m, float* d_v, int32_t step,
+                      float lr, void* stream) {
+  if (!ctx || !d_params || !d_grad || !d_m || !d_v || step < 1) return RZ_EINVAL;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+  hipLaunchKernelGGL(rz_mlp_adam_kernel, dim3((NP + 255) / 256), dim3(256), 0, (hipStream_t)stream, d_params, d_grad,
+                     d_m, d_v, lr, b1, b2, eps, c1, c2);
+  RZ_HIP(hipGetLastError());
+  return RZ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,172 @@
+"""Learned dispatch between Redzone launch forms and the O(n) partition.
+
+A small FP32 MLP (rz_mlp.hip: 12 features -> 32 -> 32 -> 32 -> 4) predicts
+log2 of the time in microseconds of each method for a list; the dispatcher
+runs the method with the smallest prediction.  Without a model it falls back
+to the fixed rule of the reference: Redzone when k < n/2, else the O(n)
+partition (README.md:6; paper p.14).
+
+Features, forward pass, gradients and Adam steps all run in the library's HIP
+kernels; this module only drives them (collects timings, batches the
+training, stores the parameters).
+
+Methods (outputs of the network):
+  0  redzone         rz_remove with the tuned launch shape (three kernels)
+  1  redzone_single  one ordered launch (roles by workgroup id)
+  2  redzone_queue   the persistent work-queue kernel with one list
+  3  partition       rz_partition_on (mark + stable O(n) compaction)
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+from . import _native
+from ._native import ListDesc, TileConfig
+
+NF = 12
+NH = 32
+NC = 4
+NP = NH * NF + NH + 2 * (NH * NH + NH) + NC * NH + NC
+assert NP == 2660
+
+METHODS = ["redzone", "redzone_single", "redzone_queue", "partition"]
+METHOD_CONFIGS = [None, TileConfig(256, 4, 4096, 3, 0, 1), TileConfig(256, 8, 0, 2, 1, 1), None]
+RULE_REDZONE, RULE_PARTITION = 0, 3
+
+MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dispatch_model_gfx950.npz")
+
+
+def rule_choice(n, k):
+    """The reference's fixed rule: Redzone is faster than O(n) when k < n/2."""
+    return RULE_REDZONE if 2 * k < n else RULE_PARTITION
+
+
+def run_method(ctx, method, data, removals, stream=None):
+    """Runs method `method` on one list (in place); returns n - k."""
+    if method == 3:
+        return ctx.partition(data, removals, stream=stream)
+    return ctx.remove(data, removals, config=METHOD_CONFIGS[method], stream=stream)
+
+
+def init_params(seed=0):
+    """He-uniform initial parameters in the library's layout (float32)."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for nin, nout in [(NF, NH), (NH, NH), (NH, NH), (NH, NC)]:
+        lim = math.sqrt(6.0 / nin)
+        parts.append(rng.uniform(-lim, lim, size=(nout, nin)).astype(np.float32).ravel())
+        parts.append(np.zeros(nout, np.float32))
+    p = np.concatenate(parts)
+    assert p.size == NP
+    return p
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Dispatcher:
+    """MLP dispatcher whose parameters live on the device."""
+
+    def __init__(self, ctx, params=None, seed=0):
+        import torch
+
+        self.ctx = ctx
+        self.lib = _native.load()
+        p = init_params(seed) if params is None else np.asarray(params, np.float32)
+        self.params = torch.from_numpy(p.copy()).cuda(ctx.device)
+        self.trained = params is not None
+        self.train_log = []
+
+    # ---- features -------------------------------------------------------
+    def features(self, lists, stream=None):
+        """lists: [(data, removals), ...] device tensors -> float32 device tensor [B, NF]."""
+        import torch
+
+        descs, cnt = self.ctx._descs(lists)
+        raw = torch.frombuffer(bytearray(ctypes.string_at(ctypes.addressof(descs), ctypes.sizeof(ListDesc) * cnt)),
+                               dtype=torch.uint8)
+        d_descs = raw.cuda(self.ctx.device)
+        out = torch.empty((cnt, NF), dtype=torch.float32, device=f"cuda:{self.ctx.device}")
+        self.ctx._call("rz_features", self.ctx.handle, _ptr(d_descs), cnt, _ptr(out),
+                       _stream_ptr(stream, self.ctx.device))
+        torch.cuda.current_stream(self.ctx.device).synchronize()
+        return out
+
+    # ---- inference ------------------------------------------------------
+    def predict(self, X, stream=None):
+        import torch
+
+        B = X.shape[0]
+        Y = torch.empty((B, NC), dtype=torch.float32, device=X.device)
+        C = torch.empty(B, dtype=torch.int32, device=X.device)
+        self.ctx._call("rz_mlp_forward", self.ctx.handle, _ptr(self.params), _ptr(X.contiguous()), B, _ptr(Y), _ptr(C),
+                       _stream_ptr(stream, self.ctx.device))
+        return Y, C
+
+    def choose(self, X, n=None, k=None):
+        """Method index per row; the k < n/2 rule when the network is untrained."""
+        if not self.trained:
+            return np.array([rule_choice(a, b) for a, b in zip(n, k)], np.int32)
+        _, C = self.predict(X)
+        return C.cpu().numpy()
+
+    # ---- training -------------------------------------------------------
+    def grad(self, X, T, M):
+        """Loss and gradient of the masked MSE on log2 times (device tensors)."""
+        import torch
+
+        B = X.shape[0]
+        grad = torch.empty(NP, dtype=torch.float32, device=X.device)
+        loss = torch.empty(1, dtype=torch.float32, device=X.device)
+        work = torch.empty(B * (NP + 1), dtype=torch.float32, device=X.device)
+        msum = float(M.sum().item())
+        self.ctx._call("rz_mlp_grad", self.ctx.handle, _ptr(self.params), _ptr(X.contiguous()), _ptr(T.contiguous()),
+                       _ptr(M.contiguous()), B, ctypes.c_float(msum), _ptr(grad), _ptr(loss), _ptr(work),
+                       _stream_ptr(None, self.ctx.device))
+        return loss, grad
+
+    def train(self, X, T, M, epochs=400, lr=3e-3, batch=256, seed=0, log_every=50):
+        """Adam on mini-batches; X [N, NF], T/M [N, NC] device float32 tensors."""
+        import torch
+
+        N = X.shape[0]
+        m = torch.zeros(NP, dtype=torch.float32, device=X.device)
+        v = torch.zeros(NP, dtype=torch.float32, device=X.device)
+        rng = np.random.default_rng(seed)
+        step = 0
+        for ep in range(epochs):
+            order = torch.from_numpy(rng.permutation(N)).to(X.device)
+            for s in range(0, N, batch):
+                idx = order[s: s + batch]
+                Xb, Tb, Mb = X[idx].contiguous(), T[idx].contiguous(), M[idx].contiguous()
+                if float(Mb.sum().item()) <= 0:
+                    continue
+                _, g = self.grad(Xb, Tb, Mb)
+                step += 1
+                self.ctx._call("rz_mlp_adam", self.ctx.handle, _ptr(self.params), _ptr(g), _ptr(m), _ptr(v), step,
+                               ctypes.c_float(lr), _stream_ptr(None, self.ctx.device))
+            if log_every and (ep % log_every == 0 or ep == epochs - 1):
+                loss, _ = self.grad(X, T, M)
+                self.train_log.append((ep, float(loss.item())))
+        self.trained = True
+        return self.train_log
+
+    # ---- persistence ----------------------------------------------------
+    def save(self, path=MODEL_PATH, meta=None):
+        np.savez(path, params=self.params.cpu().numpy(), meta=np.array(repr(meta or {})))
+
+    @classmethod
+    def load(cls, ctx, path=MODEL_PATH):
+        with np.load(path, allow_pickle=False) as z:
+            return cls(ctx, params=z["params"])
+
+
+def _stream_ptr(stream, device):
+    import torch
+
+    if stream is None:
+        stream = torch.cuda.current_stream(device)
+    return ctypes.c_void_p(stream.cuda_stream)
