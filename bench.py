@@ -50,6 +50,8 @@ def parse():
     p.add_argument("--items", type=int, default=None)
     p.add_argument("--max-slices", type=int, default=2048)
     p.add_argument("--slice-steps", type=int, default=2)
+    p.add_argument("--no-batch", action="store_true", help="skip the batched (4096-list) comparison")
+    p.add_argument("--mixed-lists", type=int, default=10000, help="lists of the mixed-stream evaluation (0 = skip)")
     return p.parse_args()
 
 
@@ -107,6 +109,89 @@ def time_calls(torch, ctx, fn_restore, fn_call, steps, warmup, scrub, profile):
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
     return times, profs, wall
+
+
+def bench_batched(torch, ctx, rz, args):
+    """Batched compaction: 4096 int32 lists, n log-uniform in [2^8, 2^20],
+    k = floor(r n) with r ~ U[0, 0.3], uniform distinct removals.  One
+    persistent launch (direct and hipGraph replay) vs 4096 plain per-list
+    launches; inputs restored before each pass; wall time with syncs."""
+    import numpy as np
+
+    rng = np.random.default_rng(4096)
+    specs = []
+    for i in range(4096):
+        n = int(2 ** rng.uniform(8, 20))
+        specs.append((n, int(rng.uniform(0, 0.3) * n)))
+    tot_n = sum(n for n, _ in specs)
+    A0 = torch.empty(tot_n, dtype=torch.int32, device="cuda")
+    ctx.gen_payload(A0, 77)
+    A = torch.empty_like(A0)
+    R0s, lists = [], []
+    off = 0
+    for i, (n, k) in enumerate(specs):
+        R = torch.empty(k, dtype=torch.int32, device="cuda")
+        ctx.gen_uniform_removals(R, n, 1000 + i)
+        R0s.append(R)
+        lists.append((A[off: off + n], torch.empty_like(R)))
+        off += n
+    sum_k = sum(k for _, k in specs)
+    sum_B = 0
+    for (n, k), R in zip(specs, R0s):
+        if 0 < k < n:
+            sum_B += 8 * k + 4 * (k - int((R >= n - k).sum().item()))
+
+    def restore():
+        A.copy_(A0)
+        for (_, r), r0 in zip(lists, R0s):
+            r.copy_(r0)
+        torch.cuda.synchronize()
+
+    def timed(fn, reps):
+        ts = []
+        for _ in range(reps + 1):
+            restore()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return statistics.median(ts[1:]) * 1e3
+
+    reps = max(3, args.steps // 4)
+    b = ctx.batch(lists)
+    info = b.info()
+    t_batch = timed(b.launch, reps)
+    s = torch.cuda.Stream()  # the legacy null stream cannot be captured
+    with torch.cuda.stream(s):
+        b.capture(stream=s)
+    t_graph = timed(lambda: b.launch(stream=s), reps)
+    b.close()
+    t_plain = timed(lambda: [ctx.remove(a, r) for a, r in lists], reps)
+    return {"lists": len(specs), "sum_n": tot_n, "sum_k": sum_k, "algorithmic_bytes": sum_B,
+            "batched_ms": round(t_batch, 4), "graph_ms": round(t_graph, 4), "plain_per_list_ms": round(t_plain, 3),
+            "batched_vs_plain": round(t_plain / min(t_batch, t_graph), 2),
+            "batched_Gitems_s": round(tot_n / (min(t_batch, t_graph) * 1e-3) / 1e9, 1),
+            "batched_GBps": round(sum_B / (min(t_batch, t_graph) * 1e-3) / 1e9, 1),
+            "queue_items": info["nitems"], "grid": info["grid"], "config": ctx.batch_config().as_tuple()}
+
+
+def bench_mixed(torch, ctx, args):
+    """Mixed stream (lists of 4/8/16-byte items, n in [2^12, 2^26], k/n in
+    [1e-5, 0.9], three removal patterns): the learned dispatcher (trained on a
+    disjoint seed) vs the k < n/2 rule and vs the measured per-list best."""
+    from redzone_amd import mixed as MX
+    from redzone_amd.dispatch import MODEL_PATH, Dispatcher
+
+    if not os.path.exists(MODEL_PATH):
+        return None
+    disp = Dispatcher.load(ctx, MODEL_PATH)
+    specs = MX.stream_specs(args.mixed_lists, seed=2)
+    res = MX.evaluate(ctx, disp, specs, 2 * 1000003, reps=3, log=log)
+    res.pop("_per_list_us")
+    res.pop("_chosen")
+    out = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res.items()}
+    out["lists"] = args.mixed_lists
+    return out
 
 
 def cpu_baseline(A0_h, R0_h, n, k, seconds):
@@ -258,6 +343,15 @@ def main():
         extra["warm_ms"] = round(statistics.median(t_warm), 4)
         result["comparators"] = extra
         log(f"comparators: {extra}")
+
+    if not args.no_batch:
+        del A, R
+        torch.cuda.empty_cache()
+        result["batched"] = bench_batched(torch, ctx, rz, args)
+        log(f"batched: {result['batched']}")
+    if args.mixed_lists > 0:
+        result["mixed"] = bench_mixed(torch, ctx, args)
+        log(f"mixed: {result['mixed']}")
 
     if not args.no_cpu:
         # restore the pristine list on the host copy for the CPU baseline

@@ -158,8 +158,9 @@ rz_status rz_batch_capture(rz_batch* b, void* stream);
 rz_status rz_batch_get_info(rz_batch* b, rz_batch_info* out);
 /* Waits for the device, then frees the batch. */
 rz_status rz_batch_destroy(rz_batch* b);
-/* One-shot: plan + launch.  The plan is kept in the context until the next
- * call (or rz_ctx_destroy), so the launch stays asynchronous. */
+/* One-shot: plan + launch on a batch owned by the context.  Its device
+ * buffers only grow and the plan is uploaded asynchronously on the stream, so
+ * repeated calls neither allocate nor synchronise in steady state. */
 rz_status rz_remove_batched(rz_ctx* ctx, const rz_list_desc* lists, int32_t nlists, uint32_t flags, void* stream);
 
 /* ---- synthetic inputs ---------------------------------------------------- */

@@ -123,14 +123,36 @@ SIGNATURES = [
 TUNED_TABLE = os.path.join(_HERE, "tuned_gfx950.txt")
 
 
+def _hip_runtimes_loaded():
+    """Paths of the HIP runtime libraries mapped into this process."""
+    try:
+        with open("/proc/self/maps") as f:
+            return sorted({line.split()[-1] for line in f if "libamdhip64" in line})
+    except OSError:
+        return []
+
+
 def load():
-    """Load librz_amd.so; raise if it has not been built."""
+    """Load librz_amd.so; raise if it has not been built.
+
+    torch ships its own HIP runtime with the same soname as the system one
+    (libamdhip64.so.7).  torch is imported first so that the library binds to
+    the runtime torch already loaded: one runtime means torch's streams and
+    device memory are this library's streams and memory.  Two runtimes in one
+    process would not share streams, so that case raises."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(LIB_PATH)
+        runtimes = _hip_runtimes_loaded()
+        if len(runtimes) > 1:
+            raise RuntimeError(f"two HIP runtimes are loaded ({runtimes}); import torch before redzone_amd")
         for name, res, args in SIGNATURES:
             fn = getattr(lib, name)
             fn.restype = res

@@ -212,17 +212,10 @@ rz_status rz_last_stats(rz_ctx* ctx, rz_stats* out) {
 
 rz_status rz_remove_batched(rz_ctx* ctx, const rz_list_desc* lists, int32_t nlists, uint32_t flags, void* stream) {
   if (!ctx) return RZ_EINVAL;
+  if (nlists < 0 || (nlists > 0 && !lists) || nlists >= (1 << 28)) return RZ_EINVAL;
   std::lock_guard<std::mutex> lock(ctx->mu);
   DeviceGuard g(ctx->device);
-  if (ctx->last_batch) {
-    rz_batch_destroy(ctx->last_batch);
-    ctx->last_batch = nullptr;
-  }
-  rz_batch* b = nullptr;
-  rz_status st = rz_batch_create(ctx, lists, nlists, flags, nullptr, &b);
-  if (st != RZ_OK) return st;
-  ctx->last_batch = b;
-  return rz_batch_launch(b, stream);
+  return batch_plan_and_launch(ctx, lists, nlists, flags, (hipStream_t)stream);
 }
 
 rz_status rz_last_profile(rz_ctx* ctx, float ms[4]) {

@@ -236,6 +236,15 @@ struct rz_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   rz_batch_info info;
+  uint32_t grid_cap = 0;  // resident workgroups of the queue kernel
+  // host-side plan and pinned staging for asynchronous uploads
+  std::vector<rz::ListPlan> h_plans;
+  std::vector<uint2> h_items;
+  std::vector<uint64_t> h_keys;
+  void* staging = nullptr;
+  size_t staging_bytes = 0;
+  hipEvent_t staged_ev = nullptr;
+  bool staged = false;
 };
 
 using namespace rz;
@@ -246,31 +255,26 @@ static void free_buf(DeviceBuffer& b) {
   b.bytes = 0;
 }
 
-extern "C" {
-
-rz_status rz_batch_create(rz_ctx* ctx, const rz_list_desc* lists, int32_t nlists, uint32_t flags,
-                          const rz_tile_config* cfg_in, rz_batch** out) {
-  if (!ctx || !out || nlists < 0 || (nlists > 0 && !lists)) return RZ_EINVAL;
-  if (nlists >= (1 << 28)) return RZ_EINVAL;
-  *out = nullptr;
-  rz_tile_config cfg = cfg_in ? *cfg_in : ctx->batch_cfg;
+// Plans the lists into batch b, reusing (growing) its device buffers.  The
+// plan reaches the device synchronously, or (async) through pinned staging on
+// stream s, ordered after the batch's previous launch on that stream.
+static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlists, uint32_t flags,
+                            const rz_tile_config& cfg, hipStream_t s, bool async) {
+  rz_ctx* ctx = b->ctx;
   QueueKernel kern = queue_kernel(cfg.wg_size, cfg.items);
   if (!kern) return RZ_EINVAL;
-  int dev_prev = -1;
-  (void)hipGetDevice(&dev_prev);
-  if (dev_prev != ctx->device) (void)hipSetDevice(ctx->device);
-
-  std::vector<ListPlan> plans((size_t)nlists);
-  std::vector<uint64_t> keys;  // (k desc, index) for LPT ordering
+  std::vector<ListPlan>& plans = b->h_plans;
+  std::vector<uint2>& items = b->h_items;
+  std::vector<uint64_t>& keys = b->h_keys;  // (k desc, index) for LPT ordering
+  plans.resize((size_t)nlists);
+  items.clear();
+  keys.clear();
   uint64_t bits_words = 0, count_words = 0, orphan_words = 0;
-  rz_status st = RZ_OK;
-  for (int32_t i = 0; i < nlists && st == RZ_OK; ++i) {
+  for (int32_t i = 0; i < nlists; ++i) {
     const rz_list_desc& d = lists[i];
     if (d.n < 0 || d.k < 0 || d.k > d.n || d.n >= (1ll << 31) || esz_index(d.elem_bytes) < 0 ||
-        (d.k > 0 && (!d.data || !d.removals))) {
-      st = RZ_EINVAL;
-      break;
-    }
+        (d.k > 0 && (!d.data || !d.removals)))
+      return RZ_EINVAL;
     fill_plan(plans[i], d.data, d.removals, d.removals, d.n, d.k, d.elem_bytes, cfg);
     plans[i].bits_off = (uint32_t)bits_words;
     plans[i].counts_off = (uint32_t)count_words;
@@ -284,13 +288,8 @@ rz_status rz_batch_create(rz_ctx* ctx, const rz_list_desc* lists, int32_t nlists
       plans[i].m = 0;  // nothing to do
     }
   }
-  if (st != RZ_OK) {
-    if (dev_prev >= 0 && dev_prev != ctx->device) (void)hipSetDevice(dev_prev);
-    return st;
-  }
   if (bits_words >= (1ull << 32) || count_words >= (1ull << 32)) return RZ_EINVAL;
   std::sort(keys.begin(), keys.end());
-  std::vector<uint2> items;
   auto push = [&](uint32_t li, uint32_t kind, uint32_t j) { items.push_back(make_uint2(li | (kind << 28), j)); };
   for (uint64_t key : keys) {
     const uint32_t li = (uint32_t)key;
@@ -311,47 +310,91 @@ rz_status rz_batch_create(rz_ctx* ctx, const rz_list_desc* lists, int32_t nlists
     if (plans[li].m > 1)
       for (uint32_t j = 0; j < plans[li].npair; ++j) push(li, kPair, j);
   }
-
-  rz_batch* b = new rz_batch();
-  b->ctx = ctx;
+  rz_status st;
+  const size_t plan_bytes = plans.size() * sizeof(ListPlan), item_bytes = items.size() * sizeof(uint2);
+  if ((st = ensure(ctx, b->plans, std::max<size_t>(64, plan_bytes), s)) != RZ_OK ||
+      (st = ensure(ctx, b->items, std::max<size_t>(8, item_bytes), s)) != RZ_OK ||
+      (st = ensure(ctx, b->qctl, (size_t)(kQWords + 2 * (size_t)nlists + 64) * 4, s)) != RZ_OK ||
+      (st = ensure(ctx, b->bits, std::max<size_t>(8, (size_t)bits_words * 4), s)) != RZ_OK ||
+      (st = ensure(ctx, b->counts, std::max<size_t>(8, (size_t)count_words * 8), s)) != RZ_OK ||
+      ((flags & RZ_PRESERVE_REMOVALS) &&
+       (st = ensure(ctx, b->orphans, std::max<size_t>(8, (size_t)orphan_words * 4), s)) != RZ_OK))
+    return st;
+  if (flags & RZ_PRESERVE_REMOVALS)
+    for (auto& P : plans) P.Rd = (uint64_t)(uintptr_t)((int32_t*)b->orphans.ptr + P.Rd);
+  if (async) {
+    const size_t need = plan_bytes + item_bytes;
+    if (b->staged) RZ_HIP(hipEventSynchronize(b->staged_ev));  // the previous upload has left the staging area
+    if (need > b->staging_bytes) {
+      if (b->staging) RZ_HIP(hipHostFree(b->staging));
+      b->staging = nullptr;
+      size_t cap = 1 << 16;
+      while (cap < need) cap *= 2;
+      if (hipHostMalloc(&b->staging, cap, hipHostMallocDefault) != hipSuccess) return RZ_ENOMEM;
+      b->staging_bytes = cap;
+    }
+    if (!b->staged_ev) RZ_HIP(hipEventCreateWithFlags(&b->staged_ev, hipEventDisableTiming));
+    memcpy(b->staging, plans.data(), plan_bytes);
+    memcpy((char*)b->staging + plan_bytes, items.data(), item_bytes);
+    if (plan_bytes) RZ_HIP(hipMemcpyAsync(b->plans.ptr, b->staging, plan_bytes, hipMemcpyHostToDevice, s));
+    if (item_bytes)
+      RZ_HIP(hipMemcpyAsync(b->items.ptr, (char*)b->staging + plan_bytes, item_bytes, hipMemcpyHostToDevice, s));
+    RZ_HIP(hipEventRecord(b->staged_ev, s));
+    b->staged = true;
+  } else {
+    if (plan_bytes) RZ_HIP(hipMemcpy(b->plans.ptr, plans.data(), plan_bytes, hipMemcpyHostToDevice));
+    if (item_bytes) RZ_HIP(hipMemcpy(b->items.ptr, items.data(), item_bytes, hipMemcpyHostToDevice));
+  }
+  if (kern != b->kern) {
+    b->kern = kern;
+    b->grid_cap = 0;
+  }
+  if (!b->grid_cap && (st = queue_grid(ctx, kern, cfg.wg_size, 0xffffffffu, &b->grid_cap)) != RZ_OK) return st;
   b->nlists = nlists;
   b->cfg = cfg;
   b->flags = flags;
-  b->kern = kern;
   b->nitems = (uint32_t)items.size();
   b->chunk = cfg.chunk > 0 ? (uint32_t)cfg.chunk : 1u;
-  hipStream_t s0 = nullptr;
-  if ((st = ensure(ctx, b->plans, std::max<size_t>(64, plans.size() * sizeof(ListPlan)), s0)) != RZ_OK ||
-      (st = ensure(ctx, b->items, std::max<size_t>(8, items.size() * sizeof(uint2)), s0)) != RZ_OK ||
-      (st = ensure(ctx, b->qctl, (size_t)(kQWords + 2 * (size_t)nlists + 64) * 4, s0)) != RZ_OK ||
-      (st = ensure(ctx, b->bits, std::max<size_t>(8, (size_t)bits_words * 4), s0)) != RZ_OK ||
-      (st = ensure(ctx, b->counts, std::max<size_t>(8, (size_t)count_words * 8), s0)) != RZ_OK ||
-      ((flags & RZ_PRESERVE_REMOVALS) &&
-       (st = ensure(ctx, b->orphans, std::max<size_t>(8, (size_t)orphan_words * 4), s0)) != RZ_OK)) {
-    rz_batch_destroy(b);
-    return st;
-  }
-  if (flags & RZ_PRESERVE_REMOVALS)
-    for (auto& P : plans) P.Rd = (uint64_t)(uintptr_t)((int32_t*)b->orphans.ptr + P.Rd);
-  if (!plans.empty() && hipMemcpy(b->plans.ptr, plans.data(), plans.size() * sizeof(ListPlan),
-                                  hipMemcpyHostToDevice) != hipSuccess) {
-    rz_batch_destroy(b);
-    return RZ_EHIP;
-  }
-  if (!items.empty() &&
-      hipMemcpy(b->items.ptr, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice) != hipSuccess) {
-    rz_batch_destroy(b);
-    return RZ_EHIP;
-  }
-  if ((st = queue_grid(ctx, kern, cfg.wg_size, b->nitems, &b->grid)) != RZ_OK) {
-    rz_batch_destroy(b);
-    return st;
-  }
+  b->grid = std::max<uint32_t>(1, std::min<uint32_t>(b->grid_cap, b->nitems));
   b->info.nitems = b->nitems;
   b->info.grid = b->grid;
   b->info.bitmap_bytes = (int64_t)bits_words * 4;
   b->info.lists_active = (int32_t)keys.size();
+  return RZ_OK;
+}
+
+// Plans into the context's reusable batch and launches it (rz_remove_batched).
+rz_status rz::batch_plan_and_launch(rz_ctx* ctx, const rz_list_desc* lists, int32_t nlists, uint32_t flags,
+                                hipStream_t s) {
+  if (!ctx->last_batch) {
+    ctx->last_batch = new rz_batch();
+    ctx->last_batch->ctx = ctx;
+  }
+  rz_status st = batch_plan(ctx->last_batch, lists, nlists, flags, ctx->batch_cfg, s, true);
+  if (st != RZ_OK) return st;
+  return rz_batch_launch(ctx->last_batch, s);
+}
+
+extern "C" {
+
+rz_status rz_batch_create(rz_ctx* ctx, const rz_list_desc* lists, int32_t nlists, uint32_t flags,
+                          const rz_tile_config* cfg_in, rz_batch** out) {
+  if (!ctx || !out || nlists < 0 || (nlists > 0 && !lists)) return RZ_EINVAL;
+  if (nlists >= (1 << 28)) return RZ_EINVAL;
+  *out = nullptr;
+  const rz_tile_config cfg = cfg_in ? *cfg_in : ctx->batch_cfg;
+  if (!queue_kernel(cfg.wg_size, cfg.items)) return RZ_EINVAL;
+  int dev_prev = -1;
+  (void)hipGetDevice(&dev_prev);
+  if (dev_prev != ctx->device) (void)hipSetDevice(ctx->device);
+  rz_batch* b = new rz_batch();
+  b->ctx = ctx;
+  const rz_status st = batch_plan(b, lists, nlists, flags, cfg, nullptr, false);
   if (dev_prev >= 0 && dev_prev != ctx->device) (void)hipSetDevice(dev_prev);
+  if (st != RZ_OK) {
+    rz_batch_destroy(b);
+    return st;
+  }
   *out = b;
   return RZ_OK;
 }
@@ -414,6 +457,8 @@ rz_status rz_batch_destroy(rz_batch* b) {
   free_buf(b->bits);
   free_buf(b->counts);
   free_buf(b->orphans);
+  if (b->staging) (void)hipHostFree(b->staging);
+  if (b->staged_ev) (void)hipEventDestroy(b->staged_ev);
   delete b;
   return RZ_OK;
 }

@@ -78,6 +78,10 @@ rz_status launch_partition(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, con
 rz_status launch_validate(rz_ctx* ctx, const int32_t* R, int64_t n, int64_t k, hipStream_t s);
 rz_status launch_queue_single(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, int32_t* R, int64_t k,
                               uint32_t flags, const rz_tile_config& cfg, hipStream_t s);
+// rz_remove_batched: plan into the context's reusable batch (buffers only grow,
+// the plan is uploaded asynchronously) and launch it.
+rz_status batch_plan_and_launch(rz_ctx* ctx, const rz_list_desc* lists, int32_t nlists, uint32_t flags,
+                                hipStream_t s);
 // RZ_PROFILE: record mark i (0..3) on the stream when profiling
 rz_status prof_mark(rz_ctx* ctx, uint32_t flags, int i, hipStream_t s);
 // fill `count` uint32 words with `v` by a kernel (graph-capture safe, unlike hipMemsetAsync)

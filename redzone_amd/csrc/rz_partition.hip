@@ -137,6 +137,155 @@ __global__ __launch_bounds__(WG) void rz_compact(T* __restrict__ A, uint32_t n, 
 }
 
 // ---------------------------------------------------------------------------
+// Vectorised form: every thread moves whole 16-byte vectors (V = 16 / sizeof(T)
+// items), the tile's bitmap words are read once into LDS (and cleared at
+// once), and a lane's rank among the kept items of a wave comes from three
+// bit-sliced ballots of its per-vector keep count.  Needs a 16-byte aligned list.
+// ---------------------------------------------------------------------------
+template <int WG, int VPT>
+struct CompactShared {
+  static constexpr int NW = WG / kWave;
+  uint32_t off[VPT * NW + 1];
+  uint32_t bits[WG * VPT * 4 / 32 + 1];  // up to 4 items per vector
+  uint32_t excl;
+};
+
+__device__ __forceinline__ uint32_t keep_count_prefix(uint32_t c, uint32_t* total) {
+  // c in [0, 4]: three bit planes
+  const uint64_t b0 = ballot(c & 1u), b1 = ballot(c & 2u), b2 = ballot(c & 4u);
+  *total = (uint32_t)__popcll(b0) + 2u * (uint32_t)__popcll(b1) + 4u * (uint32_t)__popcll(b2);
+  return lane_rank(b0) + 2u * lane_rank(b1) + 4u * lane_rank(b2);
+}
+
+template <typename T, int WG, int VPT>
+__global__ __launch_bounds__(WG) void rz_compact_vec(T* __restrict__ A, uint32_t n, uint32_t* __restrict__ bits,
+                                                      uint64_t* __restrict__ status, uint32_t* __restrict__ ctrl) {
+  constexpr int V = 16 / sizeof(T);
+  constexpr int NW = WG / kWave;
+  constexpr int E = VPT * NW;
+  constexpr int PER = (E + kWave - 1) / kWave;
+  constexpr uint32_t TILE = WG * VPT * V;
+  constexpr int WORDS = TILE / 32;
+  static_assert(TILE % 32 == 0, "tiles are bitmap-word aligned");
+  __shared__ CompactShared<WG, VPT> sh;
+  const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+  const uint32_t t = blockIdx.x;
+  const uint32_t base = t * TILE;
+  const uint32_t epoch = ld_relaxed(&ctrl[kCtrlEpoch]) & 0x3fffffffu;
+  const uint32_t wend = (n + 31u) >> 5;
+
+  // bitmap words of the tile -> LDS, and leave them zero for the next call
+  for (int w = tid; w < WORDS; w += WG) {
+    const uint32_t gw = (base >> 5) + w;
+    uint32_t v = 0xffffffffu;
+    if (gw < wend) {
+      v = bits[gw];
+      bits[gw] = 0u;
+    }
+    sh.bits[w] = v;
+  }
+  union Vec {
+    uint4 u;
+    T e[V];
+  };
+  Vec x[VPT];
+#pragma unroll
+  for (int j = 0; j < VPT; ++j) {
+    const uint32_t p0 = base + (uint32_t)(j * WG + tid) * V;
+    if (p0 + V <= n) {
+      x[j].u = *(const uint4*)(A + p0);
+    } else {
+#pragma unroll
+      for (int q = 0; q < V; ++q)
+        if (p0 + q < n) x[j].e[q] = A[p0 + q];
+    }
+  }
+  __syncthreads();
+  uint32_t keep[VPT], pre[VPT];
+#pragma unroll
+  for (int j = 0; j < VPT; ++j) {
+    const uint32_t r0 = (uint32_t)(j * WG + tid) * V;  // offset in the tile
+    const uint32_t p0 = base + r0;
+    uint32_t km = 0;
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+      const uint32_t r = r0 + q;
+      const bool dead = (sh.bits[r >> 5] >> (r & 31)) & 1u;
+      km |= (uint32_t)(!dead && p0 + q < n) << q;
+    }
+    keep[j] = km;
+    uint32_t tot;
+    pre[j] = keep_count_prefix((uint32_t)__popc(km), &tot);
+    if (lane == 0) sh.off[j * NW + wave] = tot;
+  }
+  wait_all_memory();  // every item load of this tile has returned
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t loc[PER];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int idx = lane * PER + q;
+      loc[q] = idx < E ? sh.off[idx] : 0u;
+      sum += loc[q];
+    }
+    const uint32_t xs = wave_incl_scan(sum);
+    uint32_t ex = xs - sum;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int idx = lane * PER + q;
+      if (idx < E) sh.off[idx] = ex;
+      ex += loc[q];
+    }
+    const uint32_t agg = __shfl(xs, kWave - 1, kWave);
+    uint32_t excl = 0;
+    if (t == 0) {
+      if (lane == 0) st_relaxed(&status[0], pack_status(kFlagIncl, epoch, agg));
+    } else {
+      if (lane == 0) st_relaxed(&status[t], pack_status(kFlagAgg, epoch, agg));
+      int64_t jb = (int64_t)t - 1;
+      uint32_t spins = 0;
+      while (true) {
+        const int64_t q = jb - lane;
+        uint64_t st = q >= 0 ? ld_relaxed(&status[q]) : pack_status(kFlagIncl, epoch, 0u);
+        uint32_t flag = (((uint32_t)(st >> 32) & 0x3fffffffu) == epoch) ? (uint32_t)(st >> 62) : 0u;
+        if (q < 0) flag = (uint32_t)kFlagIncl;
+        const uint64_t incl_m = ballot(flag == kFlagIncl);
+        const uint64_t none_m = ballot(flag == 0u);
+        const uint64_t upto = incl_m ? ((incl_m & (~incl_m + 1)) << 1) - 1 : ~0ull;
+        if (none_m & upto) {
+          if (++spins > (1u << 22)) {
+            if (lane == 0) or_relaxed(&ctrl[kCtrlError], kErrLookbackTimeout);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        const uint32_t v = ((upto >> lane) & 1ull) ? (uint32_t)st : 0u;
+        excl += wave_sum(v);
+        if (incl_m) break;
+        jb -= kWave;
+      }
+      if (lane == 0) st_relaxed(&status[t], pack_status(kFlagIncl, epoch, excl + agg));
+    }
+    if (lane == 0) sh.excl = excl;
+  }
+  __syncthreads();
+  const uint32_t excl = sh.excl;
+#pragma unroll
+  for (int j = 0; j < VPT; ++j) {
+    uint32_t pos = excl + sh.off[j * NW + wave] + pre[j];
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+      if ((keep[j] >> q) & 1u) {
+        if (pos < n) A[pos] = x[j].e[q];
+        ++pos;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // validation: range and duplicates (uses the n-bit bitmap, cleared afterwards)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void rz_validate_mark(const int32_t* __restrict__ R, uint32_t k, uint32_t n,
@@ -185,9 +334,16 @@ rz_status launch_validate(rz_ctx* ctx, const int32_t* R, int64_t n, int64_t k, h
 constexpr int kPartWG = 256, kPartIPT = 16;
 
 template <typename T>
+struct VecShape {
+  static constexpr int VPT = sizeof(T) == 4 ? 4 : 8;  // vectors per thread
+  static constexpr int TILE = kPartWG * VPT * (16 / (int)sizeof(T));
+};
+
+template <typename T>
 static rz_status partition_impl(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, int64_t k, uint32_t flags,
                                 hipStream_t s) {
-  constexpr int TILE = kPartWG * kPartIPT;
+  const bool vec = ((uintptr_t)A % 16) == 0;
+  const int TILE = vec ? VecShape<T>::TILE : kPartWG * kPartIPT;
   const int64_t tiles = (n + TILE - 1) / TILE;
   rz_status st;
   if ((st = ensure(ctx, ctx->list_bits, (size_t)((n + 63) / 64) * 8, s)) != RZ_OK) return st;
@@ -200,8 +356,12 @@ static rz_status partition_impl(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, 
   hipLaunchKernelGGL(rz_mark_all, g1, dim3(256), 0, s, R, (uint32_t)k, (uint32_t)n, bits, ctrl);
   RZ_HIP(hipGetLastError());
   if ((st = prof_mark(ctx, flags, 1, s)) != RZ_OK) return st;
-  hipLaunchKernelGGL((rz_compact<T, kPartWG, kPartIPT>), dim3((uint32_t)tiles), dim3(kPartWG), 0, s, A, (uint32_t)n,
-                     bits, (uint64_t*)ctx->status.ptr, ctrl);
+  if (vec)
+    hipLaunchKernelGGL((rz_compact_vec<T, kPartWG, VecShape<T>::VPT>), dim3((uint32_t)tiles), dim3(kPartWG), 0, s, A,
+                       (uint32_t)n, bits, (uint64_t*)ctx->status.ptr, ctrl);
+  else
+    hipLaunchKernelGGL((rz_compact<T, kPartWG, kPartIPT>), dim3((uint32_t)tiles), dim3(kPartWG), 0, s, A,
+                       (uint32_t)n, bits, (uint64_t*)ctx->status.ptr, ctrl);
   RZ_HIP(hipGetLastError());
   if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;
   return prof_mark(ctx, flags, 3, s);

@@ -1,0 +1,281 @@
+#!/usr/bin/env python3
+"""Mixed streams of lists: training data for the learned dispatcher and the
+mixed-stream evaluation (batched + learned dispatch vs plain per-list
+launches with the k < n/2 rule).
+
+Stream shape (one spec per list): n log-uniform in [2^12, 2^26], k/n
+log-uniform in [1e-5, 0.9] (k = floor(f n)), items of 4, 8 or 16 bytes, and
+removals uniform / clustered (geometric runs, mean 64) / tail-heavy
+(Beta(5,1)) in equal thirds.  Lists are built on the device (rz_gen_* for
+payloads and uniform removals, workloads.py torch code for the other two
+patterns) and processed in resident batches.
+
+    python -m redzone_amd.mixed train [--lists 2000] [--seed 1]   # writes the model
+    python -m redzone_amd.mixed eval  [--lists 10000] [--seed 2]  # prints one JSON line
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PATTERNS = ("uniform", "clustered", "tail_heavy")
+
+
+def stream_specs(count, seed, log2n=(12, 26), frac=(1e-5, 0.9), sizes=(4, 8, 16)):
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        n = int(2 ** rng.uniform(*log2n))
+        f = 10 ** rng.uniform(math.log10(frac[0]), math.log10(frac[1]))
+        k = int(f * n)
+        esz = int(sizes[int(rng.integers(len(sizes)))])
+        specs.append((n, k, esz, PATTERNS[i % 3]))
+    return specs
+
+
+def build_list(ctx, spec, seed):
+    """Device tensors (data, removals) of one spec; payload and removals seeded by `seed`."""
+    import torch
+
+    from . import workloads as W
+
+    n, k, esz, pattern = spec
+    shape = (n,) if esz == 4 else (n, esz // 4)
+    A = torch.empty(shape, dtype=torch.int32, device="cuda")
+    ctx.gen_payload(A, seed)
+    if pattern == "uniform":
+        R = torch.empty(k, dtype=torch.int32, device="cuda")
+        ctx.gen_uniform_removals(R, n, seed + 1)
+    elif pattern == "clustered":
+        R = W.clustered_removals(n, k, seed + 1, mean=64, device="cuda")
+    else:
+        R = W.tail_heavy_removals(n, k, seed + 1, device="cuda")
+    return A, R
+
+
+def time_methods(ctx, A0, R0, methods, reps=3):
+    """Median microseconds of each method on copies of (A0, R0), one list per launch."""
+    import torch
+
+    from .dispatch import run_method
+
+    A = torch.empty_like(A0)
+    R = torch.empty_like(R0)
+    out = []
+    for m in methods:
+        ts = []
+        for r in range(reps + 1):
+            A.copy_(A0)
+            R.copy_(R0)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            run_method(ctx, m, A, R)
+            e1.record()
+            e1.synchronize()
+            if r:
+                ts.append(e0.elapsed_time(e1) * 1e3)
+        out.append(float(np.median(ts)))
+    return np.array(out)
+
+
+def batches(specs, budget_bytes):
+    """Consecutive groups of specs whose data fits the budget."""
+    group, used = [], 0
+    for i, s in enumerate(specs):
+        b = s[0] * s[2] + s[1] * 4
+        if group and used + b > budget_bytes:
+            yield group
+            group, used = [], 0
+        group.append(i)
+        used += b
+    if group:
+        yield group
+
+
+def collect(ctx, specs, seed, reps=3, budget=8 << 30, log=None):
+    """Features [N, 12] and per-method times [N, 4] (us) of every spec."""
+    import torch
+
+    from .dispatch import METHODS, Dispatcher
+
+    disp = Dispatcher(ctx)
+    feats = np.zeros((len(specs), 12), np.float32)
+    times = np.zeros((len(specs), len(METHODS)), np.float64)
+    t0 = time.time()
+    for g in batches(specs, budget):
+        lists = [build_list(ctx, specs[i], seed + 7 * i) for i in g]
+        feats[g] = disp.features(lists).cpu().numpy()
+        for j, i in enumerate(g):
+            times[i] = time_methods(ctx, lists[j][0], lists[j][1], range(len(METHODS)), reps)
+        del lists
+        torch.cuda.empty_cache()
+        if log:
+            log(f"collected {g[-1] + 1}/{len(specs)} lists ({time.time() - t0:.0f} s)")
+    return feats, times
+
+
+def train_model(ctx, feats, times, epochs=600, lr=3e-3, seed=0):
+    import torch
+
+    from .dispatch import Dispatcher
+
+    X = torch.from_numpy(feats).cuda()
+    T = torch.from_numpy(np.log2(np.maximum(times, 0.5)).astype(np.float32)).cuda()
+    M = torch.ones_like(T)
+    disp = Dispatcher(ctx, seed=seed)
+    disp.train(X, T, M, epochs=epochs, lr=lr, batch=256, log_every=100)
+    return disp
+
+
+def run_plain(ctx, lists, specs):
+    """Per-list launches, method by the k < n/2 rule; wall time (s) from first launch to the end."""
+    import torch
+
+    from .dispatch import rule_choice, run_method
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for (A, R), (n, k, _, _) in zip(lists, specs):
+        run_method(ctx, rule_choice(n, k), A, R)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+# Lists with at least this many removals are launched on their own: their
+# time is their random writes, which the three-kernel launch handles best, while
+# the persistent queue pays off for the many small lists whose time is launch
+# overhead (a 4096-list batch costs about half a microsecond per list).
+BATCH_MAX_K = 1 << 17
+
+
+def run_batched_learned(ctx, disp, lists, specs):
+    """Features + MLP choice on the device; small Redzone lists in one
+    persistent launch, large ones with their chosen Redzone form, partition
+    lists one launch each; wall time (s) of everything including the decision
+    and the batch plan."""
+    import torch
+
+    from .dispatch import run_method
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    X = disp.features(lists)
+    choice = disp.choose(X)
+    small = [i for i in range(len(lists)) if choice[i] != 3 and specs[i][1] < BATCH_MAX_K]
+    if small:
+        ctx.remove_batched([lists[i] for i in small])
+    for i in range(len(lists)):
+        if choice[i] == 3 or specs[i][1] >= BATCH_MAX_K:
+            run_method(ctx, int(choice[i]), lists[i][0], lists[i][1])
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, choice
+
+
+def evaluate(ctx, disp, specs, seed, reps=3, budget=8 << 30, log=None):
+    """Mixed-stream evaluation; returns a dict of totals and regrets."""
+    import torch
+
+    from .dispatch import rule_choice
+
+    tot = {"plain_rule_wall_s": 0.0, "batched_learned_wall_s": 0.0, "oracle_best_us": 0.0, "learned_us": 0.0,
+           "rule_us": 0.0, "redzone_only_us": 0.0}
+    picks = np.zeros(4, np.int64)
+    per_list = np.zeros((len(specs), 4), np.float64)
+    chosen = np.zeros(len(specs), np.int32)
+    t0 = time.time()
+    for g in batches(specs, budget):
+        sub = [specs[i] for i in g]
+        pristine = [build_list(ctx, specs[i], seed + 7 * i) for i in g]
+        X = disp.features(pristine)
+        learned = disp.choose(X)
+        for j, i in enumerate(g):
+            t = time_methods(ctx, pristine[j][0], pristine[j][1], range(4), reps)
+            per_list[i] = t
+            chosen[i] = learned[j]
+            n, k = specs[i][0], specs[i][1]
+            tot["oracle_best_us"] += t.min()
+            tot["learned_us"] += t[learned[j]]
+            tot["rule_us"] += t[rule_choice(n, k)]
+            tot["redzone_only_us"] += t[0]
+            picks[learned[j]] += 1
+        work = [(torch.empty_like(A), torch.empty_like(R)) for A, R in pristine]
+
+        def restore():
+            for (a, r), (a0, r0) in zip(work, pristine):
+                a.copy_(a0)
+                r.copy_(r0)
+            torch.cuda.synchronize()
+
+        restore()
+        tot["plain_rule_wall_s"] += run_plain(ctx, work, sub)
+        restore()
+        wall, _ = run_batched_learned(ctx, disp, work, sub)
+        tot["batched_learned_wall_s"] += wall
+        del pristine, work
+        torch.cuda.empty_cache()
+        if log:
+            log(f"evaluated {g[-1] + 1}/{len(specs)} lists ({time.time() - t0:.0f} s)")
+    tot["learned_regret"] = tot["learned_us"] / tot["oracle_best_us"] - 1
+    tot["rule_regret"] = tot["rule_us"] / tot["oracle_best_us"] - 1
+    tot["batched_learned_vs_plain"] = tot["plain_rule_wall_s"] / tot["batched_learned_wall_s"]
+    tot["picks"] = {m: int(c) for m, c in zip(["redzone", "redzone_single", "redzone_queue", "partition"], picks)}
+    tot["_per_list_us"] = per_list
+    tot["_chosen"] = chosen
+    return tot
+
+
+def main():
+    import redzone_amd as rz
+
+    from .dispatch import MODEL_PATH, Dispatcher
+
+    p = argparse.ArgumentParser()
+    p.add_argument("mode", choices=["train", "eval"])
+    p.add_argument("--lists", type=int, default=None)
+    p.add_argument("--seed", type=int, default=None)
+    p.add_argument("--reps", type=int, default=3)
+    p.add_argument("--model", default=MODEL_PATH)
+    p.add_argument("--epochs", type=int, default=600)
+    p.add_argument("--details", default=None, help="eval: write per-list times and choices to this .npz")
+    args = p.parse_args()
+
+    def log(*a):
+        print(*a, file=sys.stderr, flush=True)
+
+    ctx = rz.Context(0)
+    if args.mode == "train":
+        count, seed = args.lists or 2000, 1 if args.seed is None else args.seed
+        specs = stream_specs(count, seed)
+        feats, times = collect(ctx, specs, seed * 1000003, args.reps, log=log)
+        disp = train_model(ctx, feats, times, epochs=args.epochs)
+        disp.save(args.model, meta={"lists": count, "seed": seed, "train_log": disp.train_log})
+        X = feats
+        pred, _ = disp.predict(__import__("torch").from_numpy(X).cuda())
+        choice = np.argmin(pred.cpu().numpy(), axis=1)
+        best = times.min(1).sum()
+        print(json.dumps({"mode": "train", "lists": count, "seed": seed, "train_log": disp.train_log,
+                          "train_regret": float(times[np.arange(len(specs)), choice].sum() / best - 1),
+                          "model": args.model}), flush=True)
+    else:
+        count, seed = args.lists or 10000, 2 if args.seed is None else args.seed
+        disp = Dispatcher.load(ctx, args.model)
+        specs = stream_specs(count, seed)
+        res = evaluate(ctx, disp, specs, seed * 1000003, args.reps, log=log)
+        per_list, chosen = res.pop("_per_list_us"), res.pop("_chosen")
+        if args.details:
+            np.savez(args.details, specs=np.array(specs, dtype=object).astype(str), times_us=per_list, chosen=chosen)
+        res.update({"mode": "eval", "lists": count, "seed": seed})
+        print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

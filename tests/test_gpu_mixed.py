@@ -1,0 +1,65 @@
+"""Mixed streams: the generated lists are valid, the timing collection and the
+trained dispatcher run, and every list processed by the batched + learned
+path equals the oracle (canonical Redzone result, or the exact stable order
+when the dispatcher picked the O(n) partition)."""
+import numpy as np
+import pytest
+import torch
+
+from redzone_amd import dispatch as D
+from redzone_amd import mixed as MX
+from tests.parity import announce, expected_canonical, first_mismatch
+
+pytestmark = pytest.mark.gpu
+
+
+def test_specs_and_lists_valid(ctx):
+    specs = MX.stream_specs(30, seed=9, log2n=(4, 16))
+    assert specs == MX.stream_specs(30, seed=9, log2n=(4, 16))
+    for i, spec in enumerate(specs):
+        n, k, esz, pattern = spec
+        announce("mixed list", i, spec)
+        A, R = MX.build_list(ctx, spec, 100 + i)
+        assert A.shape[0] == n and R.shape[0] == k
+        r = R.cpu().numpy()
+        assert len(np.unique(r)) == k and (k == 0 or (r.min() >= 0 and r.max() < n))
+
+
+def test_batched_learned_path_matches_oracle(ctx, oracle):
+    specs = MX.stream_specs(60, seed=11, log2n=(8, 18))
+    announce("collect")
+    feats, times = MX.collect(ctx, specs[:30], seed=500, reps=1)
+    assert np.all(times > 0)
+    announce("train")
+    disp = MX.train_model(ctx, feats, times, epochs=30)
+    lists = [MX.build_list(ctx, s, 900 + 7 * i) for i, s in enumerate(specs)]
+    host = [(A.cpu().numpy(), R.cpu().numpy()) for A, R in lists]
+    announce("run batched learned")
+    _, choice = MX.run_batched_learned(ctx, disp, lists, specs)
+    torch.cuda.synchronize()
+    for i, ((A0, R0), (A, _)) in enumerate(zip(host, lists)):
+        n, k = specs[i][0], specs[i][1]
+        got = A.cpu().numpy()
+        if choice[i] == 3:
+            bad, fb = oracle.check(A0, got, R0, mode=1)
+            assert bad == 0, (i, specs[i], bad, fb)
+        else:
+            want = expected_canonical(oracle, A0, R0)
+            m = first_mismatch(got[: n - k], want[: n - k])
+            assert m < 0, (i, specs[i], m)
+
+
+def test_plain_rule_path_matches_oracle(ctx, oracle):
+    specs = MX.stream_specs(40, seed=12, log2n=(6, 16))
+    lists = [MX.build_list(ctx, s, 300 + 7 * i) for i, s in enumerate(specs)]
+    host = [(A.cpu().numpy(), R.cpu().numpy()) for A, R in lists]
+    announce("run plain rule")
+    MX.run_plain(ctx, lists, specs)
+    for i, ((A0, R0), (A, _)) in enumerate(zip(host, lists)):
+        n, k = specs[i][0], specs[i][1]
+        got = A.cpu().numpy()
+        if D.rule_choice(n, k) == 3:
+            assert oracle.check(A0, got, R0, mode=1)[0] == 0
+        else:
+            want = expected_canonical(oracle, A0, R0)
+            assert first_mismatch(got[: n - k], want[: n - k]) < 0

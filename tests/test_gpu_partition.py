@@ -47,6 +47,25 @@ def test_partition_ragged(ctx, oracle):
         _run(ctx, oracle, A, R)
 
 
+def test_partition_unaligned_list(ctx, oracle):
+    """A list that starts one item into a buffer is not 16-byte aligned: the
+    scalar tile kernel handles it."""
+    n = (1 << 18) + 13
+    esz = 4
+    full = W.payload(esz, n + 1, seed=5)
+    R = W.uniform_removals(n, n // 7, seed=6)
+    A_np = full.numpy()[1:].copy()
+    want = A_np.copy()
+    assert oracle.remove_stable(want, R.numpy(), threads=4)[0] == 0
+    dfull = full.cuda()
+    view = dfull[1:]
+    announce("partition unaligned")
+    assert ctx.partition(view, R.cuda()) == n - R.shape[0]
+    torch.cuda.synchronize()
+    got = dfull.cpu().numpy()[1:]
+    assert np.array_equal(got[: n - R.shape[0]], want[: n - R.shape[0]])
+
+
 def test_partition_repeated_calls(ctx, oracle):
     """Back-to-back calls reuse the bitmap and the look-back state."""
     n = 1 << 20
