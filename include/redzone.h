@@ -51,6 +51,8 @@ typedef enum rz_status {
 #define RZ_COOPERATIVE 0x4u       /* single cooperative launch with in-kernel grid barriers */
 #define RZ_STATS 0x8u             /* record per-call statistics readable with rz_last_stats (synchronises) */
 #define RZ_PROFILE 0x10u          /* time each kernel with events on the stream; read with rz_last_profile */
+#define RZ_SORTED 0x20u           /* d_removals is strictly ascending: the mark step reads only its red-zone suffix
+                                     (checked by RZ_VALIDATE; if it is not true the result is unspecified) */
 
 typedef struct rz_ctx rz_ctx;
 

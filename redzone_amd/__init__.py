@@ -14,11 +14,11 @@ import ctypes
 import os
 
 from . import _native
-from ._native import (RZ_COOPERATIVE, RZ_PRESERVE_REMOVALS, RZ_PROFILE, RZ_STATS, RZ_VALIDATE, BatchInfo, ListDesc,
+from ._native import (RZ_COOPERATIVE, RZ_PRESERVE_REMOVALS, RZ_PROFILE, RZ_SORTED, RZ_STATS, RZ_VALIDATE, BatchInfo, ListDesc,
                       RedzoneError, Stats, TileConfig)
 
 __all__ = ["Context", "Batch", "remove", "partition", "TileConfig", "Stats", "RedzoneError", "RZ_PRESERVE_REMOVALS",
-           "RZ_VALIDATE", "RZ_COOPERATIVE", "RZ_STATS", "RZ_PROFILE", "elem_bytes_of", "default_context"]
+           "RZ_VALIDATE", "RZ_COOPERATIVE", "RZ_STATS", "RZ_PROFILE", "RZ_SORTED", "elem_bytes_of", "default_context"]
 
 
 def elem_bytes_of(data):

@@ -17,6 +17,7 @@ RZ_VALIDATE = 0x2
 RZ_COOPERATIVE = 0x4
 RZ_STATS = 0x8
 RZ_PROFILE = 0x10
+RZ_SORTED = 0x20
 
 
 class TileConfig(ctypes.Structure):

@@ -168,7 +168,7 @@ rz_status rz_remove_config(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_b
   DeviceGuard g(ctx->device);
   hipStream_t s = (hipStream_t)stream;
   if (flags & RZ_VALIDATE) {
-    if ((st = launch_validate(ctx, d_removals, n, k, s)) != RZ_OK) return st;
+    if ((st = launch_validate(ctx, d_removals, n, k, flags, s)) != RZ_OK) return st;
   }
   st = launch_plain(ctx, d_data, n, elem_bytes, d_removals, k, flags, *cfg, s);
   if (st == RZ_OK && (flags & RZ_STATS)) {
@@ -239,7 +239,7 @@ rz_status rz_partition_on(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_by
   DeviceGuard g(ctx->device);
   hipStream_t s = (hipStream_t)stream;
   if (flags & RZ_VALIDATE) {
-    if ((st = launch_validate(ctx, d_removals, n, k, s)) != RZ_OK) return st;
+    if ((st = launch_validate(ctx, d_removals, n, k, flags, s)) != RZ_OK) return st;
   }
   return launch_partition(ctx, d_data, n, elem_bytes, d_removals, k, flags, s);
 }

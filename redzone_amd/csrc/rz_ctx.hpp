@@ -18,7 +18,7 @@ enum CtrlWord : int {
   kCtrlBarEpoch = 96,     // launch epoch of the in-kernel grid barrier
   kCtrlStatTail = 128,    // uint64: tail removals k_z (RZ_STATS)
   kCtrlStatOrphans = 160, // uint64: orphan pairs (RZ_STATS)
-  kCtrlValidate = 192,    // validation flags: 1 = out of range, 2 = duplicate
+  kCtrlValidate = 192,    // validation flags: 1 = out of range, 2 = duplicate, 4 = not ascending
   kCtrlWords = 256
 };
 
@@ -75,7 +75,7 @@ rz_status launch_plain(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, int32_t
                        const rz_tile_config& cfg, hipStream_t s);
 rz_status launch_partition(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, const int32_t* R, int64_t k,
                            uint32_t flags, hipStream_t s);
-rz_status launch_validate(rz_ctx* ctx, const int32_t* R, int64_t n, int64_t k, hipStream_t s);
+rz_status launch_validate(rz_ctx* ctx, const int32_t* R, int64_t n, int64_t k, uint32_t flags, hipStream_t s);
 rz_status launch_queue_single(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, int32_t* R, int64_t k,
                               uint32_t flags, const rz_tile_config& cfg, hipStream_t s);
 // rz_remove_batched: plan into the context's reusable batch (buffers only grow,

@@ -289,10 +289,12 @@ __global__ __launch_bounds__(WG) void rz_compact_vec(T* __restrict__ A, uint32_t
 // validation: range and duplicates (uses the n-bit bitmap, cleared afterwards)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void rz_validate_mark(const int32_t* __restrict__ R, uint32_t k, uint32_t n,
-                                                         uint32_t* __restrict__ bits, uint32_t* __restrict__ flag) {
+                                                         uint32_t* __restrict__ bits, uint32_t* __restrict__ flag,
+                                                         int check_sorted) {
   uint32_t bad = 0;
   for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < k; i += gridDim.x * 256u) {
     const uint32_t r = (uint32_t)R[i];
+    if (check_sorted && i + 1 < k && (uint32_t)R[i + 1] <= r) bad |= 4u;
     if (r >= n) { bad |= 1u; continue; }
     const uint32_t bit = 1u << (r & 31);
     if (or_relaxed(&bits[r >> 5], bit) & bit) bad |= 2u;
@@ -308,7 +310,7 @@ __global__ __launch_bounds__(256) void rz_validate_clear(const int32_t* __restri
   }
 }
 
-rz_status launch_validate(rz_ctx* ctx, const int32_t* R, int64_t n, int64_t k, hipStream_t s) {
+rz_status launch_validate(rz_ctx* ctx, const int32_t* R, int64_t n, int64_t k, uint32_t flags, hipStream_t s) {
   rz_status st;
   if ((st = ensure(ctx, ctx->list_bits, (size_t)((n + 63) / 64) * 8, s)) != RZ_OK) return st;
   uint32_t* ctrl = (uint32_t*)ctx->ctrl.ptr;
@@ -316,7 +318,7 @@ rz_status launch_validate(rz_ctx* ctx, const int32_t* R, int64_t n, int64_t k, h
   const int64_t gw = (k + 255) / 256;
   const dim3 g((uint32_t)(gw < (int64_t)ctx->cu_count * 8 ? (gw > 0 ? gw : 1) : ctx->cu_count * 8));
   hipLaunchKernelGGL(rz_validate_mark, g, dim3(256), 0, s, R, (uint32_t)k, (uint32_t)n,
-                     (uint32_t*)ctx->list_bits.ptr, ctrl + kCtrlValidate);
+                     (uint32_t*)ctx->list_bits.ptr, ctrl + kCtrlValidate, (flags & RZ_SORTED) ? 1 : 0);
   hipLaunchKernelGGL(rz_validate_clear, g, dim3(256), 0, s, R, (uint32_t)k, (uint32_t)n,
                      (uint32_t*)ctx->list_bits.ptr);
   RZ_HIP(hipGetLastError());
@@ -325,6 +327,7 @@ rz_status launch_validate(rz_ctx* ctx, const int32_t* R, int64_t n, int64_t k, h
   const uint32_t f = ctx->ctrl_host[kCtrlValidate];
   if (f & 1u) return RZ_ERANGE;
   if (f & 2u) return RZ_EDUP;
+  if (f & 4u) return RZ_EINVAL;  // RZ_SORTED given for a list that is not ascending
   return RZ_OK;
 }
 

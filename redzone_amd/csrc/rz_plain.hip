@@ -41,6 +41,18 @@ __global__ __launch_bounds__(256) void rz_mark_tail(const int32_t* __restrict__ 
   mark_body<STATS>(R, k, k, sZ, bits, stat_tail, blockIdx.x * 256u + threadIdx.x, gridDim.x * 256u);
 }
 
+// Sorted removal list: the removals that fall in the red zone are R's suffix.
+// Workgroup w takes the chunk R[w*C, (w+1)*C) and leaves at once when the
+// chunk's last entry is below the red zone, so only the suffix is read.
+constexpr uint32_t kSortedChunk = 4096;
+__global__ __launch_bounds__(256) void rz_mark_tail_sorted(const int32_t* __restrict__ R, uint32_t k, uint32_t sZ,
+                                                            uint32_t* __restrict__ bits) {
+  const uint32_t lo = blockIdx.x * kSortedChunk;
+  const uint32_t hi = min(lo + kSortedChunk, k);
+  if ((uint32_t)R[hi - 1] < sZ) return;  // uniform across the workgroup
+  mark_body<false>(R + lo, hi - lo, k, sZ, bits, nullptr, threadIdx.x, 256u);
+}
+
 template <typename T, int WG, int IPT>
 __global__ __launch_bounds__(WG) void rz_move(T* __restrict__ A, const int32_t* __restrict__ R,
                                                int32_t* __restrict__ Rd, uint32_t k, uint32_t sZ, uint32_t L,
@@ -275,6 +287,9 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
   if (stats)
     hipLaunchKernelGGL(rz_mark_tail<true>, g1, dim3(256), 0, s, R, (uint32_t)k, sZ, bits,
                        (uint64_t*)(ctrl + kCtrlStatTail));
+  else if (flags & RZ_SORTED)
+    hipLaunchKernelGGL(rz_mark_tail_sorted, dim3((uint32_t)((k + kSortedChunk - 1) / kSortedChunk)), dim3(256), 0,
+                       s, R, (uint32_t)k, sZ, bits);
   else
     hipLaunchKernelGGL(rz_mark_tail<false>, g1, dim3(256), 0, s, R, (uint32_t)k, sZ, bits, (uint64_t*)nullptr);
   RZ_HIP(hipGetLastError());

@@ -189,6 +189,31 @@ def test_validate_rejects_bad_lists(ctx, oracle):
     gpu_remove_parity(ctx, oracle, A, R, flags=rz.RZ_VALIDATE)
 
 
+@pytest.mark.parametrize("n,k,pattern", [(1 << 20, 10_000, "uniform"), (1 << 20, 4096 * 3, "tail_heavy"),
+                                         (1 << 18, (1 << 18) // 2 + 7, "clustered"), (70_001, 1, "uniform")])
+def test_sorted_flag(ctx, oracle, n, k, pattern):
+    """RZ_SORTED reads only the red-zone suffix of an ascending list in the mark
+    step; the survivors must be the same as without the flag."""
+    announce("sorted", n, k, pattern)
+    A = W.unique_u32_payload(n, seed=91)
+    R = getattr(W, f"{pattern}_removals")(n, k, seed=92)
+    R = torch.sort(R).values.contiguous()
+    gpu_remove_parity(ctx, oracle, A, R, flags=rz.RZ_SORTED)
+    gpu_remove_parity(ctx, oracle, A, R, flags=rz.RZ_SORTED | rz.RZ_VALIDATE)
+
+
+def test_sorted_flag_validate_rejects_unsorted(ctx):
+    n = 5000
+    A = W.unique_u32_payload(n, seed=93)
+    dA = A.cuda()
+    announce("sorted validate")
+    with pytest.raises(rz.RedzoneError) as e:
+        ctx.remove(dA, torch.tensor([5, 900, 17], dtype=torch.int32).cuda(), flags=rz.RZ_SORTED | rz.RZ_VALIDATE)
+    assert e.value.status == 1
+    torch.cuda.synchronize()
+    assert torch.equal(dA.cpu(), A)
+
+
 def test_stats(ctx, oracle):
     n = 1 << 18
     k = n // 10
