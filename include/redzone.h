@@ -61,8 +61,11 @@ typedef struct rz_ctx rz_ctx;
 typedef struct rz_tile_config {
   int32_t wg_size;     /* 256 or 512 */
   int32_t items;       /* items per thread per step: 2, 4, 8, 16 (variants 0/1); 4 or 8 (variant 2, batches) */
-  int32_t max_slices;  /* variants 0/1: upper bound on slices (workgroups of the move step) */
-  int32_t variant;     /* 0 = three kernels on the stream, 1 = one cooperative kernel, 2 = persistent queue */
+  int32_t max_slices;  /* variants 0/1/3/4: upper bound on slices (workgroups of the move step) */
+  int32_t variant;     /* 0 = three kernels on the stream (mark, move, pair), 1 = one cooperative kernel,
+                          2 = persistent queue, 3 = one ordered launch (roles by workgroup id),
+                          4 = two kernels (mark, then move with in-kernel orphan pairing; slices longer than
+                              the workgroup LDS allows run as variant 0) */
   int32_t slice_steps; /* variant 2 and batches: steps per slice (0 = 4) */
   int32_t chunk;       /* variant 2 and batches: queue items taken per ticket (0 = 1) */
 } rz_tile_config;

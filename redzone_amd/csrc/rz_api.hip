@@ -97,6 +97,7 @@ rz_status rz_ctx_create(rz_ctx** out, int32_t device) {
   rz_ctx* ctx = new rz_ctx();
   ctx->device = device;
   ctx->cu_count = prop.multiProcessorCount;
+  ctx->lds_per_block = prop.sharedMemPerBlock < 65536 ? (int)prop.sharedMemPerBlock : 65536;
   strncpy(ctx->arch, prop.gcnArchName, sizeof(ctx->arch) - 1);
   void* p = nullptr;
   if (hipMalloc(&p, kCtrlWords * 4) != hipSuccess) { delete ctx; return RZ_ENOMEM; }
@@ -124,7 +125,7 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
   (void)hipDeviceSynchronize();
   if (ctx->last_batch) rz_batch_destroy(ctx->last_batch);
   DeviceBuffer* bufs[] = {&ctx->tail_bits, &ctx->list_bits, &ctx->counts, &ctx->orphans,
-                          &ctx->status,    &ctx->ctrl,      &ctx->queue};
+                          &ctx->status,    &ctx->ctrl,      &ctx->queue,  &ctx->mstatus, &ctx->slots};
   for (DeviceBuffer* b : bufs)
     if (b->ptr) (void)hipFree(b->ptr);
   if (ctx->ctrl_host) (void)hipHostFree(ctx->ctrl_host);

@@ -19,6 +19,7 @@ enum CtrlWord : int {
   kCtrlStatTail = 128,    // uint64: tail removals k_z (RZ_STATS)
   kCtrlStatOrphans = 160, // uint64: orphan pairs (RZ_STATS)
   kCtrlValidate = 192,    // validation flags: 1 = out of range, 2 = duplicate, 4 = not ascending
+  kCtrlMoveEpoch = 224,   // call counter of the fused move-and-pair form (bumped by its mark kernel)
   kCtrlWords = 256
 };
 
@@ -42,6 +43,7 @@ struct rz_ctx {
   int cu_count = 0;
   char arch[64] = {0};
   int max_wg_per_cu_256 = 0;  // occupancy of the cooperative kernel at 256 threads
+  int lds_per_block = 65536;  // LDS bytes a workgroup may use (capped at 64 KiB)
   // scratch, all zero-initialised on allocation and kept zero by the kernels
   rz::DeviceBuffer tail_bits;  // k bits: tail removals of the plain path
   rz::DeviceBuffer list_bits;  // n bits: removals of the O(n) path and validation
@@ -50,6 +52,8 @@ struct rz_ctx {
   rz::DeviceBuffer status;     // look-back status words of the partition
   rz::DeviceBuffer ctrl;       // control block (rz::CtrlWord)
   rz::DeviceBuffer queue;      // ticket / exit / done words of the single-list queue launch
+  rz::DeviceBuffer mstatus;    // fused form: two halves of kMaxSlices look-back words, used alternately
+  rz::DeviceBuffer slots;      // fused form: one exchange word per orphan rank (epoch | position + 1)
   rz_tile_config batch_cfg;    // default launch shape of batches
   rz_batch* last_batch = nullptr;  // plan of the last rz_remove_batched call
   uint32_t* ctrl_host = nullptr;  // pinned mirror for stats / validation reads

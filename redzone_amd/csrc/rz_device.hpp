@@ -53,6 +53,9 @@ __device__ __forceinline__ uint64_t add_relaxed(uint64_t* p, uint64_t v) {
 __device__ __forceinline__ uint32_t add_relaxed(uint32_t* p, uint32_t v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ uint64_t xchg_relaxed(uint64_t* p, uint64_t v) {
+  return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Wait until every memory operation this wave issued has completed (gfx9
 // s_waitcnt with all counters at zero; on gfx950 vmcnt also counts stores).

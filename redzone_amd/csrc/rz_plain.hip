@@ -25,7 +25,9 @@
 // boundary being the grid-wide barrier; (1) one cooperative kernel with
 // in-kernel barriers made of relaxed atomics (RemoveFromList_GPUWide's
 // structure, redzone.hxx:626-941); (2) the persistent work-queue kernel of
-// rz_batch.hip with a single list.
+// rz_batch.hip with a single list; (3) one ordered launch, roles by workgroup
+// id; (4) mark, then one kernel that moves and pairs orphans through a
+// decoupled look-back and per-rank exchange words (rz_move_pair).
 //
 // Contract violations (duplicates, out-of-range indices) never cause an
 // out-of-bounds access: marks are taken only for indices inside the tail,
@@ -35,9 +37,12 @@
 
 namespace rz {
 
+// `epoch` (fused form only, else null) is bumped once per call.
 template <bool STATS>
 __global__ __launch_bounds__(256) void rz_mark_tail(const int32_t* __restrict__ R, uint32_t k, uint32_t sZ,
-                                                     uint32_t* __restrict__ bits, uint64_t* __restrict__ stat_tail) {
+                                                     uint32_t* __restrict__ bits, uint64_t* __restrict__ stat_tail,
+                                                     uint32_t* __restrict__ epoch) {
+  if (epoch && blockIdx.x == 0 && threadIdx.x == 0) add_relaxed(epoch, 1u);
   mark_body<STATS>(R, k, k, sZ, bits, stat_tail, blockIdx.x * 256u + threadIdx.x, gridDim.x * 256u);
 }
 
@@ -46,7 +51,8 @@ __global__ __launch_bounds__(256) void rz_mark_tail(const int32_t* __restrict__ 
 // chunk's last entry is below the red zone, so only the suffix is read.
 constexpr uint32_t kSortedChunk = 4096;
 __global__ __launch_bounds__(256) void rz_mark_tail_sorted(const int32_t* __restrict__ R, uint32_t k, uint32_t sZ,
-                                                            uint32_t* __restrict__ bits) {
+                                                            uint32_t* __restrict__ bits, uint32_t* __restrict__ epoch) {
+  if (epoch && blockIdx.x == 0 && threadIdx.x == 0) add_relaxed(epoch, 1u);
   const uint32_t lo = blockIdx.x * kSortedChunk;
   const uint32_t hi = min(lo + kSortedChunk, k);
   if ((uint32_t)R[hi - 1] < sZ) return;  // uniform across the workgroup
@@ -67,6 +73,195 @@ __global__ __launch_bounds__(256) void rz_pair(T* __restrict__ A, const int32_t*
                                                 uint32_t* __restrict__ ctrl, int stats) {
   __shared__ PairShared<256, kMaxSlices> sh;
   pair_body<T, 256, kMaxSlices, false>(A, Rd, sZ, L, ns, counts, ctrl, stats, blockIdx.x, gridDim.x, sh);
+}
+
+// ---------------------------------------------------------------------------
+// fused move + pair (variant 4): mark, then one kernel that moves and pairs.
+//
+// The move step writes no orphans to R or Z.  A slice keeps its orphans in
+// LDS (dest positions from the front of an L-entry list, source tail indices
+// from the back; a pair yields at most one orphan, so L entries suffice),
+// publishes its two orphan counts, and learns the counts of all earlier
+// slices by decoupled look-back over per-slice status words.  Slices are
+// dispatched in id order and never wait on a later slice, so the look-back
+// always finishes.  Orphan rank t then meets its partner through one exchange
+// word per rank: each side swaps in (epoch << 32 | own position + 1); the side
+// that gets back its partner's entry of the same epoch performs the move
+// A[dest] = Z[src] and clears the word.  The t-th dest orphan still pairs with
+// the t-th source orphan in i order (redzone.hxx:834-934), so the array is the
+// same as with the three-kernel form.  Nothing in R and nothing in the red
+// zone is written, so RZ_PRESERVE_REMOVALS costs nothing here, and no step
+// waits for its predecessor's scattered stores to drain.
+//
+// Status word: [63:62] flag (0 none, 1 aggregate, 2 inclusive prefix),
+// [61:31] dest orphans, [30:0] source orphans (each count is below 2^31, so
+// packed words add without carries between the fields).  The words live in
+// two halves of kMaxSlices used on alternate calls (parity of the epoch that
+// the mark kernel bumps); a call zeroes the half it does not use, so the next
+// call starts from zeros.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kStAgg = 1ull << 62, kStIncl = 2ull << 62, kStCount = (1ull << 62) - 1;
+
+template <typename T, int WG, int IPT>
+__global__ __launch_bounds__(WG) void rz_move_pair(T* __restrict__ A, const int32_t* __restrict__ R, uint32_t k,
+                                                    uint32_t sZ, uint32_t L, uint32_t ns,
+                                                    uint32_t* __restrict__ bits, uint64_t* __restrict__ mstatus,
+                                                    uint64_t* __restrict__ slots, uint32_t slot_cap,
+                                                    uint32_t* __restrict__ ctrl, int stats) {
+  constexpr int NW = WG / kWave;
+  constexpr int E = MoveShared<WG, IPT>::E;
+  constexpr int PER = (E + kWave - 1) / kWave;
+  __shared__ MoveShared<WG, IPT> sh;
+  __shared__ uint64_t s_excl;
+  extern __shared__ uint32_t s_orph[];  // L entries
+  uint32_t* s_off = sh.off;
+  const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+  const uint32_t slice = blockIdx.x;
+  const uint32_t a = slice * L, b = min(a + L, k);
+  T* __restrict__ Z = A + sZ;
+  const uint32_t epoch = ld_relaxed(&ctrl[kCtrlMoveEpoch]);
+  uint64_t* __restrict__ st = mstatus + (size_t)(epoch & 1u) * kMaxSlices;
+  {
+    uint64_t* other = mstatus + (size_t)((epoch & 1u) ^ 1u) * kMaxSlices;
+    for (uint32_t w = slice * WG + tid; w < (uint32_t)kMaxSlices; w += ns * WG) other[w] = 0ull;
+  }
+
+  uint32_t carry_s = 0, carry_d = 0;
+  for (uint32_t base = a; base < b; base += WG * IPT) {
+    uint32_t d[IPT], wv[IPT], info[IPT];
+    T z[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t p = base + j * WG + tid;
+      if (p < b) {
+        d[j] = (uint32_t)R[p];
+        z[j] = Z[p];
+        wv[j] = bits[p >> 5];
+      } else {
+        d[j] = sZ;           // invalid destination
+        wv[j] = 0xffffffffu; // dead source
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t p = base + j * WG + tid;
+      const bool dead = (wv[j] >> (p & 31)) & 1u;
+      const bool dst_ok = d[j] < sZ;
+      const uint32_t cls = dead ? (dst_ok ? kDstOrphan : kNone) : (dst_ok ? kMove : kSrcOrphan);
+      const uint64_t ms = ballot(cls == kSrcOrphan);
+      const uint64_t md = ballot(cls == kDstOrphan);
+      info[j] = cls | (lane_rank(ms) << 8) | (lane_rank(md) << 16);
+      if (lane == 0) s_off[j * NW + wave] = (uint32_t)__popcll(ms) | ((uint32_t)__popcll(md) << 16);
+    }
+    __syncthreads();
+    if (wave == 0) {
+      uint32_t loc[PER];
+      uint32_t sum = 0;
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int idx = lane * PER + q;
+        loc[q] = idx < E ? s_off[idx] : 0u;
+        sum += loc[q];
+      }
+      const uint32_t x = wave_incl_scan(sum);
+      uint32_t ex = x - sum;
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int idx = lane * PER + q;
+        if (idx < E) s_off[idx] = ex;
+        ex += loc[q];
+      }
+      if (lane == kWave - 1) {
+        s_off[E] = x;
+        if (base + WG * IPT >= b) {
+          // the slice's counts are final: publish them before the last moves
+          const uint64_t agg = ((uint64_t)(carry_d + (x >> 16)) << 31) | (uint64_t)(carry_s + (x & 0xffffu));
+          st_relaxed(&st[slice], (slice == 0 ? kStIncl : kStAgg) | agg);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t cls = info[j] & 0xffu;
+      const uint32_t off = s_off[j * NW + wave];
+      if (cls == kDstOrphan)
+        s_orph[carry_d + (off >> 16) + (info[j] >> 16)] = d[j];
+      else if (cls == kSrcOrphan)
+        s_orph[L - 1u - (carry_s + (off & 0xffffu) + ((info[j] >> 8) & 0xffu))] = base + j * WG + tid;
+    }
+    const uint32_t tot = s_off[E];
+    carry_s += tot & 0xffffu;
+    carry_d += tot >> 16;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j)
+      if ((info[j] & 0xffu) == kMove) A[d[j]] = z[j];
+    __syncthreads();
+  }
+
+  // exclusive orphan counts of the earlier slices
+  if (wave == 0) {
+    const uint64_t agg = ((uint64_t)carry_d << 31) | carry_s;
+    uint64_t excl = 0;
+    if (slice > 0) {
+      int64_t j = (int64_t)slice - 1;
+      uint32_t spins = 0;
+      while (true) {
+        const int64_t q = j - lane;
+        const uint64_t w = q >= 0 ? ld_relaxed(&st[q]) : kStIncl;
+        const uint32_t flag = (uint32_t)(w >> 62);
+        const uint64_t incl_m = ballot(flag == 2u);
+        const uint64_t none_m = ballot(flag == 0u);
+        // lanes up to and including the nearest inclusive prefix must all be ready
+        const uint64_t upto = incl_m ? ((incl_m & (~incl_m + 1)) << 1) - 1 : ~0ull;
+        if (none_m & upto) {
+          if (++spins > (1u << 24)) {
+            if (lane == 0) or_relaxed(&ctrl[kCtrlError], kErrLookbackTimeout);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        excl += wave_sum(((upto >> lane) & 1ull) ? (w & kStCount) : 0ull);
+        if (incl_m) break;
+        j -= kWave;
+      }
+      if (lane == 0) st_relaxed(&st[slice], kStIncl | (excl + agg));
+    }
+    if (lane == 0) {
+      s_excl = excl;
+      if (slice == ns - 1) {
+        const uint64_t tot = excl + agg;
+        const uint32_t td = (uint32_t)(tot >> 31), ts = (uint32_t)tot & 0x7fffffffu;
+        if (td != ts) or_relaxed(&ctrl[kCtrlError], kErrOrphanMismatch);
+        if (stats) st_relaxed((uint64_t*)&ctrl[kCtrlStatOrphans], (uint64_t)min(td, ts));
+      }
+    }
+  }
+  __syncthreads();
+
+  // meet the partners
+  const uint64_t excl = s_excl;
+  const uint32_t xd = (uint32_t)(excl >> 31), xs = (uint32_t)excl & 0x7fffffffu;
+  const uint64_t tag = (uint64_t)epoch << 32;
+  const uint32_t norph = carry_d + carry_s;
+  for (uint32_t e = tid; e < norph; e += WG) {
+    const bool is_d = e < carry_d;
+    const uint32_t mine = is_d ? s_orph[e] : s_orph[L - 1u - (e - carry_d)];
+    const uint32_t t = is_d ? xd + e : xs + (e - carry_d);
+    if (t >= slot_cap) continue;  // only with an invalid removal list
+    const uint64_t old = xchg_relaxed(&slots[t], tag | (uint64_t)(mine + 1u));
+    if ((uint32_t)(old >> 32) != epoch || (uint32_t)old == 0u) continue;  // first to arrive
+    const uint32_t other = (uint32_t)old - 1u;
+    if (is_d) {
+      if (other < k) A[mine] = Z[other];
+    } else {
+      if (other < sZ) A[other] = Z[mine];
+    }
+    st_relaxed(&slots[t], 0ull);
+  }
+  // leave the bitmap zero for the next call (slices are word aligned)
+  for (uint32_t w = (a >> 5) + tid; w < ((b + 31u) >> 5); w += WG) bits[w] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -213,6 +408,30 @@ static bool dispatch_move(int wg, int ipt, dim3 grid, hipStream_t s, T* A, const
   return false;
 }
 
+template <typename T>
+static bool dispatch_move_pair(int wg, int ipt, hipStream_t s, T* A, const int32_t* R, uint32_t k, uint32_t sZ,
+                               uint32_t L, uint32_t ns, uint32_t* bits, uint64_t* mstatus, uint64_t* slots,
+                               uint32_t slot_cap, uint32_t* ctrl, int stats) {
+  const size_t lds = (size_t)L * 4;
+#define RZ_MP_CASE(W, I)                                                                                     \
+  if (wg == W && ipt == I) {                                                                                 \
+    hipLaunchKernelGGL((rz_move_pair<T, W, I>), dim3(ns), dim3(W), lds, s, A, R, k, sZ, L, ns, bits, mstatus, \
+                       slots, slot_cap, ctrl, stats);                                                        \
+    return true;                                                                                             \
+  }
+  RZ_MP_CASE(256, 2) RZ_MP_CASE(256, 4) RZ_MP_CASE(256, 8) RZ_MP_CASE(256, 16)
+  RZ_MP_CASE(512, 2) RZ_MP_CASE(512, 4) RZ_MP_CASE(512, 8) RZ_MP_CASE(512, 16)
+#undef RZ_MP_CASE
+  return false;
+}
+
+// The fused form keeps a slice's orphan list in LDS (4 bytes per pair); a
+// slice too long for the device's per-workgroup LDS runs the three-kernel form.
+static bool fused_fits(const rz_ctx* ctx, const rz_tile_config& cfg, uint32_t L) {
+  const size_t fixed = 4096;  // static LDS of the kernel, rounded up
+  return (size_t)L * 4 + fixed <= (size_t)ctx->lds_per_block && cfg.wg_size * cfg.items <= (int)L;
+}
+
 void slice_shape(int64_t k, const rz_tile_config& cfg, int max_slices_cap, uint32_t* L, uint32_t* ns) {
   const int64_t step = (int64_t)cfg.wg_size * cfg.items;
   int64_t cap = cfg.max_slices < 1 ? 1 : cfg.max_slices;
@@ -268,7 +487,10 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
   uint64_t* counts = (uint64_t*)ctx->counts.ptr;
   uint32_t* ctrl = (uint32_t*)ctx->ctrl.ptr;
   const int stats = (flags & RZ_STATS) ? 1 : 0;
-  if (stats && (st = fill_u32(ctrl + kCtrlStatTail, 0u, 2, s)) != RZ_OK) return st;
+  if (stats) {  // RZ_STATS reports the counts and the error bits of this call only
+    if ((st = fill_u32(ctrl + kCtrlStatTail, 0u, 2, s)) != RZ_OK) return st;
+    if ((st = fill_u32(ctrl + kCtrlError, 0u, 1, s)) != RZ_OK) return st;
+  }
   if ((st = prof_mark(ctx, flags, 0, s)) != RZ_OK) return st;
   if (coop || cfg.variant == 3) {
     if ((st = prof_mark(ctx, flags, 1, s)) != RZ_OK) return st;
@@ -282,18 +504,35 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
     if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;
     return prof_mark(ctx, flags, 3, s);
   }
+  const bool fused = cfg.variant == 4 && fused_fits(ctx, cfg, L);
+  uint32_t* epoch = nullptr;
+  if (fused) {
+    if ((st = ensure(ctx, ctx->mstatus, 2 * (size_t)kMaxSlices * 8, s)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->slots, ((size_t)k / 2 + 1) * 8, s)) != RZ_OK) return st;
+    epoch = ctrl + kCtrlMoveEpoch;
+  }
   const int64_t g1w = (k + 256 * 4 - 1) / (256 * 4);
   const dim3 g1((uint32_t)(g1w < (int64_t)ctx->cu_count * 8 ? g1w : ctx->cu_count * 8));
   if (stats)
     hipLaunchKernelGGL(rz_mark_tail<true>, g1, dim3(256), 0, s, R, (uint32_t)k, sZ, bits,
-                       (uint64_t*)(ctrl + kCtrlStatTail));
+                       (uint64_t*)(ctrl + kCtrlStatTail), epoch);
   else if (flags & RZ_SORTED)
     hipLaunchKernelGGL(rz_mark_tail_sorted, dim3((uint32_t)((k + kSortedChunk - 1) / kSortedChunk)), dim3(256), 0,
-                       s, R, (uint32_t)k, sZ, bits);
+                       s, R, (uint32_t)k, sZ, bits, epoch);
   else
-    hipLaunchKernelGGL(rz_mark_tail<false>, g1, dim3(256), 0, s, R, (uint32_t)k, sZ, bits, (uint64_t*)nullptr);
+    hipLaunchKernelGGL(rz_mark_tail<false>, g1, dim3(256), 0, s, R, (uint32_t)k, sZ, bits, (uint64_t*)nullptr,
+                       epoch);
   RZ_HIP(hipGetLastError());
   if ((st = prof_mark(ctx, flags, 1, s)) != RZ_OK) return st;
+  if (fused) {
+    if (!dispatch_move_pair<T>(cfg.wg_size, cfg.items, s, A, R, (uint32_t)k, sZ, L, ns, bits,
+                               (uint64_t*)ctx->mstatus.ptr, (uint64_t*)ctx->slots.ptr, (uint32_t)(k / 2 + 1), ctrl,
+                               stats))
+      return RZ_EINVAL;
+    RZ_HIP(hipGetLastError());
+    if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;
+    return prof_mark(ctx, flags, 3, s);
+  }
   if (!dispatch_move<T>(cfg.wg_size, cfg.items, dim3(ns), s, A, R, Rd, (uint32_t)k, sZ, L, bits, counts))
     return RZ_EINVAL;
   RZ_HIP(hipGetLastError());
