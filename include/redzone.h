@@ -65,8 +65,11 @@ typedef struct rz_tile_config {
   int32_t variant;     /* 0 = three kernels on the stream (mark, move, pair), 1 = one cooperative kernel,
                           2 = persistent queue, 3 = one ordered launch (roles by workgroup id),
                           4 = two kernels (mark, then move with in-kernel orphan pairing; slices longer than
-                              the workgroup LDS allows run as variant 0) */
-  int32_t slice_steps; /* variant 2 and batches: steps per slice (0 = 4) */
+                              the workgroup LDS allows run as variant 0),
+                          5 = dense form: head writes grouped by destination bin, then one scatter per bin
+                              (for k/n of several percent and up) */
+  int32_t slice_steps; /* variant 2 and batches: steps per slice (0 = 4); variant 5: log2 of the target
+                          number of destination bins, 5..10 (0 = 9) */
   int32_t chunk;       /* variant 2 and batches: queue items taken per ticket (0 = 1) */
 } rz_tile_config;
 

@@ -125,7 +125,8 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
   (void)hipDeviceSynchronize();
   if (ctx->last_batch) rz_batch_destroy(ctx->last_batch);
   DeviceBuffer* bufs[] = {&ctx->tail_bits, &ctx->list_bits, &ctx->counts, &ctx->orphans,
-                          &ctx->status,    &ctx->ctrl,      &ctx->queue,  &ctx->mstatus, &ctx->slots};
+                          &ctx->status,    &ctx->ctrl,      &ctx->queue,  &ctx->mstatus, &ctx->slots,
+                          &ctx->dense_cnt, &ctx->dense_small, &ctx->dense_ent, &ctx->dense_src};
   for (DeviceBuffer* b : bufs)
     if (b->ptr) (void)hipFree(b->ptr);
   if (ctx->ctrl_host) (void)hipHostFree(ctx->ctrl_host);

@@ -54,6 +54,10 @@ struct rz_ctx {
   rz::DeviceBuffer queue;      // ticket / exit / done words of the single-list queue launch
   rz::DeviceBuffer mstatus;    // fused form: two halves of kMaxSlices look-back words, used alternately
   rz::DeviceBuffer slots;      // fused form: one exchange word per orphan rank (epoch | position + 1)
+  rz::DeviceBuffer dense_cnt;    // dense form: per-tile bin counts, then their per-bin prefixes
+  rz::DeviceBuffer dense_small;  // dense form: bin totals / bases, per-tile orphan counts / ranks
+  rz::DeviceBuffer dense_ent;    // dense form: bin-grouped head writes (positions, then values)
+  rz::DeviceBuffer dense_src;    // dense form: source orphan values by rank
   rz_tile_config batch_cfg;    // default launch shape of batches
   rz_batch* last_batch = nullptr;  // plan of the last rz_remove_batched call
   uint32_t* ctrl_host = nullptr;  // pinned mirror for stats / validation reads
@@ -86,6 +90,12 @@ rz_status launch_queue_single(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, 
 // the plan is uploaded asynchronously) and launch it.
 rz_status batch_plan_and_launch(rz_ctx* ctx, const rz_list_desc* lists, int32_t nlists, uint32_t flags,
                                 hipStream_t s);
+// slices of the move step: L pairs each (a multiple of wg_size*items), ns = ceil(k/L) <= max_slices_cap
+void slice_shape(int64_t k, const rz_tile_config& cfg, int max_slices_cap, uint32_t* L, uint32_t* ns);
+// dense form (rz_dense.hip): every step after the tail marks
+template <typename T>
+rz_status dense_moves(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, int64_t k, const rz_tile_config& cfg,
+                      uint32_t* bits, uint32_t* ctrl, int stats, hipStream_t s);
 // RZ_PROFILE: record mark i (0..3) on the stream when profiling
 rz_status prof_mark(rz_ctx* ctx, uint32_t flags, int i, hipStream_t s);
 // fill `count` uint32 words with `v` by a kernel (graph-capture safe, unlike hipMemsetAsync)

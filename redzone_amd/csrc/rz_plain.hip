@@ -524,6 +524,11 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
                        epoch);
   RZ_HIP(hipGetLastError());
   if ((st = prof_mark(ctx, flags, 1, s)) != RZ_OK) return st;
+  if (cfg.variant == 5) {
+    if ((st = dense_moves<T>(ctx, A, n, R, k, cfg, bits, ctrl, stats, s)) != RZ_OK) return st;
+    if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;
+    return prof_mark(ctx, flags, 3, s);
+  }
   if (fused) {
     if (!dispatch_move_pair<T>(cfg.wg_size, cfg.items, s, A, R, (uint32_t)k, sZ, L, ns, bits,
                                (uint64_t*)ctx->mstatus.ptr, (uint64_t*)ctx->slots.ptr, (uint32_t)(k / 2 + 1), ctrl,

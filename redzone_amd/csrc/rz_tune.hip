@@ -56,6 +56,8 @@ static const rz_tile_config kCandidates[] = {
     {256, 4, 4096, 3, 0, 1},  {256, 8, 0, 2, 1, 1},     {512, 8, 0, 2, 1, 1},     {256, 4, 0, 2, 2, 1},
     {256, 4, 512, 4, 0, 1},   {256, 4, 2048, 4, 0, 1},  {256, 8, 4096, 4, 0, 1},  {512, 8, 1024, 4, 0, 1},
     {512, 8, 4096, 4, 0, 1},  {256, 2, 4096, 4, 0, 1},
+    {512, 8, 1024, 5, 9, 1},  {512, 8, 2048, 5, 10, 1}, {1024, 8, 512, 5, 9, 1}, {1024, 4, 4096, 5, 10, 1},
+    {256, 8, 4096, 5, 8, 1},
 };
 
 static const rz_tile_config kBatchCandidates[] = {

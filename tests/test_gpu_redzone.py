@@ -155,6 +155,46 @@ def test_fused_variant_flags(ctx, oracle):
     gpu_remove_parity(ctx, oracle, A, R, config=rz.TileConfig(256, 4, 1, 4, 0, 0))
 
 
+@pytest.mark.parametrize("esz", [4, 8, 16])
+def test_dense_variant(ctx, oracle, esz):
+    """Region-bucketed form: head writes grouped by destination bin, dest orphans
+    carrying their rank, then a per-bin scatter: the same array again."""
+    cases = [((1 << 18) + 5, 0.01, 4096, 9, "uniform"), ((1 << 18) + 5, 0.4, 64, 5, "tail_heavy"),
+             (1 << 16, 0.9, 7, 10, "clustered"), (1 << 20, 0.25, 4096, 9, "uniform"), (3000, 0.5, 4096, 9, "uniform"),
+             (70000, 0.001, 1, 6, "uniform"), ((1 << 21) + 77, 0.5, 512, 10, "clustered"),
+             ((1 << 20) + 3, 0.1, 1024, 8, "tail_heavy")]
+    for n, frac, ms, lgb, pattern in cases:
+        k = int(n * frac)
+        A = _payload(esz, n, seed=39)
+        R = W.removals(pattern, n, k, seed=40)
+        want = expected_canonical(oracle, A.numpy(), R.numpy())
+        for wg, it in ((256, 4), (512, 8), (1024, 4)):
+            cfg = rz.TileConfig(wg, it, ms, 5, lgb, 0)
+            announce("dense", esz, n, k, pattern, cfg)
+            gpu_remove_parity(ctx, oracle, A, R, config=cfg, want=want)
+
+
+def test_dense_variant_flags(ctx, oracle):
+    n = 1 << 20
+    k = n // 4
+    A = W.index_payload(n)
+    R = W.uniform_removals(n, k, seed=41)
+    cfg = rz.TileConfig(512, 8, 1024, 5, 9, 0)
+    announce("dense stats")
+    _, dR = gpu_remove_parity(ctx, oracle, A, R, config=cfg, flags=rz.RZ_STATS | rz.RZ_PRESERVE_REMOVALS)
+    s = ctx.last_stats()
+    Rn = R.numpy()
+    sz = n - k
+    dead = np.zeros(k, bool)
+    dead[Rn[Rn >= sz] - sz] = True
+    assert s.tail_removals == int((Rn >= sz).sum())
+    assert s.orphan_pairs == int(((Rn < sz) & dead).sum())
+    assert s.error_flags == 0
+    assert torch.equal(dR.cpu(), R)
+    announce("dense sorted")
+    gpu_remove_parity(ctx, oracle, A, torch.sort(R).values.contiguous(), config=cfg, flags=rz.RZ_SORTED)
+
+
 def test_ordered_variant_stats_and_preserve(ctx, oracle):
     n = 1 << 18
     k = n // 10
