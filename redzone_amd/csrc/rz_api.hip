@@ -98,6 +98,7 @@ rz_status rz_ctx_create(rz_ctx** out, int32_t device) {
   ctx->device = device;
   ctx->cu_count = prop.multiProcessorCount;
   ctx->lds_per_block = prop.sharedMemPerBlock < 65536 ? (int)prop.sharedMemPerBlock : 65536;
+  ctx->lds_max = (int)prop.sharedMemPerBlock;
   strncpy(ctx->arch, prop.gcnArchName, sizeof(ctx->arch) - 1);
   void* p = nullptr;
   if (hipMalloc(&p, kCtrlWords * 4) != hipSuccess) { delete ctx; return RZ_ENOMEM; }
@@ -126,7 +127,8 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
   if (ctx->last_batch) rz_batch_destroy(ctx->last_batch);
   DeviceBuffer* bufs[] = {&ctx->tail_bits, &ctx->list_bits, &ctx->counts, &ctx->orphans,
                           &ctx->status,    &ctx->ctrl,      &ctx->queue,  &ctx->mstatus, &ctx->slots,
-                          &ctx->dense_cnt, &ctx->dense_small, &ctx->dense_ent, &ctx->dense_src};
+                          &ctx->dense_cnt, &ctx->dense_small, &ctx->dense_ent, &ctx->dense_fine,
+                          &ctx->dense_src};
   for (DeviceBuffer* b : bufs)
     if (b->ptr) (void)hipFree(b->ptr);
   if (ctx->ctrl_host) (void)hipHostFree(ctx->ctrl_host);

@@ -43,7 +43,8 @@ struct rz_ctx {
   int cu_count = 0;
   char arch[64] = {0};
   int max_wg_per_cu_256 = 0;  // occupancy of the cooperative kernel at 256 threads
-  int lds_per_block = 65536;  // LDS bytes a workgroup may use (capped at 64 KiB)
+  int lds_per_block = 65536;  // LDS bytes the fused form's orphan list may use (capped at 64 KiB)
+  int lds_max = 65536;        // LDS bytes one workgroup may use on this device
   // scratch, all zero-initialised on allocation and kept zero by the kernels
   rz::DeviceBuffer tail_bits;  // k bits: tail removals of the plain path
   rz::DeviceBuffer list_bits;  // n bits: removals of the O(n) path and validation
@@ -56,7 +57,8 @@ struct rz_ctx {
   rz::DeviceBuffer slots;      // fused form: one exchange word per orphan rank (epoch | position + 1)
   rz::DeviceBuffer dense_cnt;    // dense form: per-tile bin counts, then their per-bin prefixes
   rz::DeviceBuffer dense_small;  // dense form: bin totals / bases, per-tile orphan counts / ranks
-  rz::DeviceBuffer dense_ent;    // dense form: bin-grouped head writes (positions, then values)
+  rz::DeviceBuffer dense_ent;    // dense form: coarse-bin head writes (positions, then values)
+  rz::DeviceBuffer dense_fine;   // dense form: fine-bin head writes (fixed capacity per bin)
   rz::DeviceBuffer dense_src;    // dense form: source orphan values by rank
   rz_tile_config batch_cfg;    // default launch shape of batches
   rz_batch* last_batch = nullptr;  // plan of the last rz_remove_batched call
@@ -95,7 +97,9 @@ void slice_shape(int64_t k, const rz_tile_config& cfg, int max_slices_cap, uint3
 // dense form (rz_dense.hip): every step after the tail marks
 template <typename T>
 rz_status dense_moves(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, int64_t k, const rz_tile_config& cfg,
-                      uint32_t* bits, uint32_t* ctrl, int stats, hipStream_t s);
+                      uint32_t* bits, uint32_t* ctrl, int stats, bool bucket_marks, hipStream_t s);
+// dense form: whether it sets the tail marks itself (bucketed) instead of the atomic mark kernel
+bool dense_bucket_marks(const rz_tile_config& cfg, int64_t k, uint32_t flags);
 // RZ_PROFILE: record mark i (0..3) on the stream when profiling
 rz_status prof_mark(rz_ctx* ctx, uint32_t flags, int i, hipStream_t s);
 // fill `count` uint32 words with `v` by a kernel (graph-capture safe, unlike hipMemsetAsync)

@@ -2,54 +2,112 @@
 //
 // A scattered 4-byte write costs one read-modify-write of a DRAM burst, so the
 // move step of the plain forms runs at about 25 G holes/s whatever the
-// density.  When holes are dense (k/n of several percent and up) most bursts
-// hold several holes, and writing them while the burst's line sits in one
-// L2 turns many read-modify-writes into one.  This form therefore regroups
-// the writes by destination before issuing them:
+// density, and so does the atomic tail marking.  When removals are dense
+// (k/n of several percent and up) most bursts hold several targets, and
+// writing them while the burst's line sits in one L2 turns many
+// read-modify-writes into one.  This form regroups both the tail marks and the
+// head writes by address before issuing them, in streaming passes:
 //
-//   mark        tail bitmap (rz_mark_tail, as in the other forms)
-//   count       tile j of L pairs: per destination bin, the number of head
-//               writes (moves and dest orphans), and the tile's dest / source
-//               orphan counts.  Bins are 2^BS consecutive positions of [0, n-k).
-//   colscan     per bin, exclusive prefix of the counts over tiles
-//   topscan     bin bases; per tile, exclusive dest / source orphan ranks
-//   entries     tile j again: each head write goes to its bin's run in
-//               (d, value) arrays; a dest orphan carries its rank t instead of
-//               a value (top bit of d set); the t-th source orphan's value goes
-//               to src[t].  Orphan ranks follow i, so pairing stays canonical
-//               (redzone.hxx:834-934).
-//   scatter     each bin is split among G workgroups laid out so that all of
-//               them share one XCD (workgroups are dispatched to the 8 XCDs
-//               round robin); A[d] = value, or src[t] for a dest orphan.
+//   tail marks (cfg.chunk == 2; otherwise the atomic rz_mark_tail)
+//     tcount    tile j of R: per tail bin (2^19 tail positions), the number
+//               of removals inside the red zone
+//     colscan / topscan   bin bases and per-tile offsets
+//     tentries  the tile's red-zone removals, staged in LDS by bin, written
+//               as contiguous runs
+//     tmark     one workgroup per tail bin sets the bin's bits in a 64 KiB
+//               LDS bitmap and writes its words whole
+//   head writes
+//     count     tile j: per coarse destination bin, the number of head writes
+//               (moves and dest orphans) and the tile's dest / source orphan
+//               counts
+//     colscan / topscan   coarse bin bases, per-tile offsets, per-tile orphan
+//               ranks (exclusive, in i order)
+//     entries   tile j again, in steps staged through LDS: (position, value)
+//               runs per coarse bin; a dest orphan carries its rank t instead
+//               of a value (top bit of the position set), and the t-th source
+//               orphan's value goes to src[t].  Orphan ranks follow i, so the
+//               pairing is the canonical one (redzone.hxx:834-934).
+//     refine    chunks of the coarse runs, staged through LDS by fine bin
+//               (about 1 MiB of the list each); runs go to fixed-capacity fine
+//               bins through one atomic cursor add per (chunk, fine bin).
+//               Distinct removals write each position at most once, so a fine
+//               bin never holds more entries than it has positions.
+//     scatter   each fine bin is split among G workgroups laid out to share
+//               one XCD (workgroups are dispatched to the 8 XCDs round robin),
+//               bins taken in order, so each XCD writes about two bins at a
+//               time and the writes combine in its L2: A[d] = value, or src[t]
+//               for a dest orphan.
 //
-// The counts and offsets are recomputed on every call, so no scratch needs
-// clearing except the tail bitmap, which the entries kernel clears as it
-// reads it.  Distinct removals give at most one head write per position, so
-// every bin run and every index stays inside its buffer; with an invalid
-// list the two passes still classify identically, so indices stay in bounds
-// and the orphan-count mismatch is flagged.
+// Counts and offsets are recomputed on every call.  The fine cursors are
+// zeroed by the count kernel of the next call; the tail bitmap is overwritten
+// whole by tmark (or set by rz_mark_tail into a zero bitmap) and cleared by
+// the entries kernel as it reads it.  With an invalid list the passes still
+// classify identically, every index is checked against its buffer, and the
+// orphan-count mismatch (or a fine-bin overflow) is flagged.
 #include "rz_redzone_dev.hpp"
 
 namespace rz {
 
-constexpr int kDenseMaxBins = 1024;
 constexpr uint32_t kDenseOrphan = 0x80000000u;
+constexpr int kCoarseMax = 1024;      // coarse destination bins and tail bins
+constexpr int kFineMax = 4096;        // fine destination bins
+constexpr uint32_t kTailBinLog = 19;  // tail positions per tail bin (a 64 KiB LDS bitmap)
+constexpr uint32_t kCoarseTarget = 64;
 
 __device__ __forceinline__ uint32_t classify(uint32_t d, bool dead, uint32_t sZ) {
   const bool dst_ok = d < sZ;
   return dead ? (dst_ok ? kDstOrphan : kNone) : (dst_ok ? kMove : kSrcOrphan);
 }
 
-// ---- count ----------------------------------------------------------------
+// Exclusive scan of a[0, N) in LDS, in place (N <= kFineMax); returns the total.
+template <int WG>
+__device__ uint32_t lds_excl_scan(uint32_t* a, uint32_t N, uint32_t* tmp) {
+  constexpr uint32_t MAXPER = kFineMax / WG;
+  const uint32_t per = (N + WG - 1) / WG;
+  const uint32_t t = threadIdx.x;
+  uint32_t loc[MAXPER], sum = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < MAXPER; ++q) {
+    const uint32_t idx = t * per + q;
+    loc[q] = (q < per && idx < N) ? a[idx] : 0u;
+    sum += loc[q];
+  }
+  uint32_t total;
+  uint32_t ex = block_excl_scan<WG>(sum, tmp, &total);
+#pragma unroll
+  for (uint32_t q = 0; q < MAXPER; ++q) {
+    const uint32_t idx = t * per + q;
+    if (q < per && idx < N) {
+      a[idx] = ex;
+      ex += loc[q];
+    }
+  }
+  __syncthreads();
+  return total;
+}
+
+template <typename T> __device__ __forceinline__ T rank_value(uint32_t t);
+template <> __device__ __forceinline__ uint32_t rank_value<uint32_t>(uint32_t t) { return t; }
+template <> __device__ __forceinline__ uint64_t rank_value<uint64_t>(uint32_t t) { return t; }
+template <> __device__ __forceinline__ uint4 rank_value<uint4>(uint32_t t) { return make_uint4(t, 0u, 0u, 0u); }
+template <typename T> __device__ __forceinline__ uint32_t value_rank(const T& v);
+template <> __device__ __forceinline__ uint32_t value_rank<uint32_t>(const uint32_t& v) { return v; }
+template <> __device__ __forceinline__ uint32_t value_rank<uint64_t>(const uint64_t& v) { return (uint32_t)v; }
+template <> __device__ __forceinline__ uint32_t value_rank<uint4>(const uint4& v) { return v.x; }
+
+// ---- count ------------------------------------------------------------------
+// Also zeroes the fine-bin cursors that this call's refine kernel adds to.
 template <int WG, int IPT>
 __global__ __launch_bounds__(WG) void rz_dense_count(const int32_t* __restrict__ R, uint32_t k, uint32_t sZ,
                                                       uint32_t L, const uint32_t* __restrict__ bits, uint32_t NB,
                                                       uint32_t bs, uint32_t* __restrict__ cnt,
-                                                      uint64_t* __restrict__ ocnt) {
-  __shared__ uint32_t hist[kDenseMaxBins];
+                                                      uint64_t* __restrict__ ocnt, uint32_t* __restrict__ fcur,
+                                                      uint32_t NBf) {
+  __shared__ uint32_t hist[kCoarseMax];
   __shared__ uint32_t s_nd, s_ns;
   const uint32_t tid = threadIdx.x, j = blockIdx.x;
   const uint32_t a = j * L, b = min(a + L, k);
+  for (uint32_t q = j * WG + tid; q < NBf; q += gridDim.x * WG) fcur[q] = 0u;
   for (uint32_t q = tid; q < NB; q += WG) hist[q] = 0u;
   if (tid == 0) { s_nd = 0u; s_ns = 0u; }
   __syncthreads();
@@ -59,12 +117,13 @@ __global__ __launch_bounds__(WG) void rz_dense_count(const int32_t* __restrict__
 #pragma unroll
     for (int q = 0; q < IPT; ++q) {
       const uint32_t p = base + q * WG + tid;
+      d[q] = sZ;
+      wv[q] = 0xffffffffu;
       if (p < b) { d[q] = (uint32_t)R[p]; wv[q] = bits[p >> 5]; }
     }
 #pragma unroll
     for (int q = 0; q < IPT; ++q) {
       const uint32_t p = base + q * WG + tid;
-      if (p >= b) continue;
       const uint32_t cls = classify(d[q], (wv[q] >> (p & 31)) & 1u, sZ);
       if (cls == kMove || cls == kDstOrphan) atomicAdd(&hist[d[q] >> bs], 1u);
       nd += cls == kDstOrphan;
@@ -127,6 +186,7 @@ __global__ __launch_bounds__(1024) void rz_dense_topscan(const uint32_t* __restr
     if (t < NB) binbase[t] = ex;
     if (t == 0) binbase[NB] = total;
   }
+  if (!obase) return;
   uint64_t carry = 0;
   for (uint32_t c0 = 0; c0 < nt; c0 += 1024u * 4u) {
     uint64_t loc[4], sum = 0;
@@ -154,16 +214,8 @@ __global__ __launch_bounds__(1024) void rz_dense_topscan(const uint32_t* __restr
   }
 }
 
-// ---- entries ----------------------------------------------------------------
-template <typename T> __device__ __forceinline__ T rank_value(uint32_t t);
-template <> __device__ __forceinline__ uint32_t rank_value<uint32_t>(uint32_t t) { return t; }
-template <> __device__ __forceinline__ uint64_t rank_value<uint64_t>(uint32_t t) { return t; }
-template <> __device__ __forceinline__ uint4 rank_value<uint4>(uint32_t t) { return make_uint4(t, 0u, 0u, 0u); }
-template <typename T> __device__ __forceinline__ uint32_t value_rank(const T& v);
-template <> __device__ __forceinline__ uint32_t value_rank<uint32_t>(const uint32_t& v) { return v; }
-template <> __device__ __forceinline__ uint32_t value_rank<uint64_t>(const uint64_t& v) { return (uint32_t)v; }
-template <> __device__ __forceinline__ uint32_t value_rank<uint4>(const uint4& v) { return v.x; }
-
+// ---- entries: staged (position, value) runs per coarse bin ------------------
+// Dynamic LDS: S values, S positions, S bin ids (S = WG * IPT).
 template <typename T, int WG, int IPT>
 __global__ __launch_bounds__(WG) void rz_dense_entries(const T* __restrict__ A, const int32_t* __restrict__ R,
                                                         uint32_t k, uint32_t sZ, uint32_t L,
@@ -176,31 +228,39 @@ __global__ __launch_bounds__(WG) void rz_dense_entries(const T* __restrict__ A, 
   constexpr int NW = WG / kWave;
   constexpr int E = MoveShared<WG, IPT>::E;
   constexpr int PER = (E + kWave - 1) / kWave;
+  constexpr uint32_t S = WG * IPT;
+  extern __shared__ __align__(16) unsigned char dyn[];
+  T* sv = (T*)dyn;
+  uint32_t* sd = (uint32_t*)(sv + S);
+  uint16_t* sb = (uint16_t*)(sd + S);
   __shared__ MoveShared<WG, IPT> sh;
-  __shared__ uint32_t cur[kDenseMaxBins];
+  __shared__ uint32_t lcnt[kCoarseMax + 1], gb[kCoarseMax], gcur[kCoarseMax];
+  __shared__ uint32_t tmp[WG / kWave + 1];
   uint32_t* s_off = sh.off;
   const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
   const uint32_t j = blockIdx.x;
   const uint32_t a = j * L, b = min(a + L, k);
   const T* __restrict__ Z = A + sZ;
-  for (uint32_t q = tid; q < NB; q += WG) cur[q] = binbase[q] + off[(size_t)j * NB + q];
+  for (uint32_t q = tid; q < NB; q += WG) {
+    gcur[q] = binbase[q] + off[(size_t)j * NB + q];
+    lcnt[q] = 0u;
+  }
   const uint64_t ob = obase[j];
   const uint32_t xd = (uint32_t)ob, xs = (uint32_t)(ob >> 32);
   __syncthreads();
   uint32_t carry_s = 0, carry_d = 0;
-  for (uint32_t base = a; base < b; base += WG * IPT) {
-    uint32_t d[IPT], wv[IPT], info[IPT];
+  for (uint32_t base = a; base < b; base += S) {
+    uint32_t d[IPT], wv[IPT], info[IPT], rk[IPT];
     T z[IPT];
 #pragma unroll
     for (int q = 0; q < IPT; ++q) {
       const uint32_t p = base + q * WG + tid;
+      d[q] = sZ;
+      wv[q] = 0xffffffffu;
       if (p < b) {
         d[q] = (uint32_t)R[p];
         z[q] = Z[p];
         wv[q] = bits[p >> 5];
-      } else {
-        d[q] = sZ;
-        wv[q] = 0xffffffffu;
       }
     }
 #pragma unroll
@@ -211,6 +271,7 @@ __global__ __launch_bounds__(WG) void rz_dense_entries(const T* __restrict__ A, 
       const uint64_t md = ballot(cls == kDstOrphan);
       info[q] = cls | (lane_rank(ms) << 8) | (lane_rank(md) << 16);
       if (lane == 0) s_off[q * NW + wave] = (uint32_t)__popcll(ms) | ((uint32_t)__popcll(md) << 16);
+      rk[q] = (cls == kMove || cls == kDstOrphan) ? atomicAdd(&lcnt[d[q] >> bs], 1u) : 0u;
     }
     __syncthreads();
     if (wave == 0) {
@@ -232,7 +293,13 @@ __global__ __launch_bounds__(WG) void rz_dense_entries(const T* __restrict__ A, 
       }
       if (lane == kWave - 1) s_off[E] = x;
     }
+    // reserve this step's run of every bin
+    for (uint32_t q = tid; q < NB; q += WG) {
+      gb[q] = gcur[q];
+      gcur[q] += lcnt[q];
+    }
     __syncthreads();
+    const uint32_t total = lds_excl_scan<WG>(lcnt, NB, tmp);
 #pragma unroll
     for (int q = 0; q < IPT; ++q) {
       const uint32_t cls = info[q] & 0xffu;
@@ -241,37 +308,127 @@ __global__ __launch_bounds__(WG) void rz_dense_entries(const T* __restrict__ A, 
         const uint32_t t = xs + carry_s + (o & 0xffffu) + ((info[q] >> 8) & 0xffu);
         if (t < src_cap) src[t] = z[q];
       } else if (cls != kNone) {
-        const uint32_t slot = atomicAdd(&cur[d[q] >> bs], 1u);
+        const uint32_t bin = d[q] >> bs;
+        const uint32_t e = lcnt[bin] + rk[q];
         const bool mv = cls == kMove;
         const uint32_t t = xd + carry_d + (o >> 16) + (info[q] >> 16);
-        ent_d[slot] = mv ? d[q] : (d[q] | kDenseOrphan);
-        ent_v[slot] = mv ? z[q] : rank_value<T>(t);
+        sd[e] = mv ? d[q] : (d[q] | kDenseOrphan);
+        sv[e] = mv ? z[q] : rank_value<T>(t);
+        sb[e] = (uint16_t)bin;
       }
     }
     const uint32_t tot = s_off[E];
     carry_s += tot & 0xffffu;
     carry_d += tot >> 16;
     __syncthreads();
+    for (uint32_t e = tid; e < total; e += WG) {
+      const uint32_t bin = sb[e];
+      const uint32_t g = gb[bin] + (e - lcnt[bin]);
+      ent_d[g] = sd[e];
+      ent_v[g] = sv[e];
+    }
+    __syncthreads();
+    for (uint32_t q = tid; q < NB; q += WG) lcnt[q] = 0u;
+    __syncthreads();
   }
   // leave the tail bitmap zero for the next call (tiles are word aligned)
   for (uint32_t w = (a >> 5) + tid; w < ((b + 31u) >> 5); w += WG) bits[w] = 0u;
 }
 
+// ---- refine: coarse runs -> fixed-capacity fine bins -------------------------
+// Chunk c = entries [c*S, (c+1)*S) of the coarse arrays (the grid is sized for
+// k entries; chunks past the total leave at once).  Fine bin f holds the
+// entries of positions [f << bsf, (f+1) << bsf) at fent[(f << bsf) + ...].
+template <typename T, int WG, int IPT>
+__global__ __launch_bounds__(WG) void rz_dense_refine(const uint32_t* __restrict__ ent_d,
+                                                       const T* __restrict__ ent_v,
+                                                       const uint32_t* __restrict__ total_p,
+                                                       uint32_t* __restrict__ fent_d, T* __restrict__ fent_v,
+                                                       uint32_t* __restrict__ fcur, uint32_t NBf, uint32_t bsf,
+                                                       uint32_t* __restrict__ ctrl) {
+  constexpr uint32_t S = WG * IPT;
+  extern __shared__ __align__(16) unsigned char dyn[];
+  T* sv = (T*)dyn;
+  uint32_t* sd = (uint32_t*)(sv + S);
+  uint16_t* sb = (uint16_t*)(sd + S);
+  __shared__ uint32_t lcnt[kFineMax + 1], gb[kFineMax];
+  __shared__ uint32_t tmp[WG / kWave + 1];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t total = *total_p;
+  const uint32_t c0 = blockIdx.x * S;
+  if (c0 >= total) return;  // uniform across the workgroup
+  const uint32_t c1 = min(c0 + S, total);
+  for (uint32_t q = tid; q < NBf; q += WG) lcnt[q] = 0u;
+  __syncthreads();
+  uint32_t d[IPT], rk[IPT];
+  T v[IPT];
+#pragma unroll
+  for (int q = 0; q < IPT; ++q) {
+    const uint32_t e = c0 + q * WG + tid;
+    d[q] = 0xffffffffu;
+    v[q] = T();
+    if (e < c1) {
+      d[q] = ent_d[e];
+      v[q] = ent_v[e];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < IPT; ++q) {
+    const uint32_t f = (d[q] & ~kDenseOrphan) >> bsf;
+    rk[q] = (d[q] != 0xffffffffu && f < NBf) ? atomicAdd(&lcnt[f], 1u) : 0xffffffffu;
+  }
+  __syncthreads();
+  for (uint32_t q = tid; q < NBf; q += WG) {
+    const uint32_t cq = lcnt[q];
+    gb[q] = cq ? add_relaxed(&fcur[q], cq) : 0u;
+  }
+  __syncthreads();
+  const uint32_t tot = lds_excl_scan<WG>(lcnt, NBf, tmp);
+#pragma unroll
+  for (int q = 0; q < IPT; ++q) {
+    if (rk[q] != 0xffffffffu) {
+      const uint32_t f = (d[q] & ~kDenseOrphan) >> bsf;
+      const uint32_t e = lcnt[f] + rk[q];
+      sd[e] = d[q];
+      sv[e] = v[q];
+      sb[e] = (uint16_t)f;
+    }
+  }
+  __syncthreads();
+  const uint32_t cap = 1u << bsf;
+  bool over = false;
+  for (uint32_t e = tid; e < tot; e += WG) {
+    const uint32_t f = sb[e];
+    const uint32_t r = gb[f] + (e - lcnt[f]);
+    if (r < cap) {
+      const size_t g = ((size_t)f << bsf) + r;
+      fent_d[g] = sd[e];
+      fent_v[g] = sv[e];
+    } else {
+      over = true;
+    }
+  }
+  if (over) or_relaxed(&ctrl[kCtrlError], kErrOrphanMismatch);
+}
+
 // ---- scatter ----------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(1024) void rz_dense_scatter(T* __restrict__ A, const uint32_t* __restrict__ ent_d,
-                                                        const T* __restrict__ ent_v,
-                                                        const uint32_t* __restrict__ binbase, uint32_t G,
-                                                        const T* __restrict__ src, uint32_t src_cap, uint32_t sZ) {
+__global__ __launch_bounds__(1024) void rz_dense_scatter(T* __restrict__ A, const uint32_t* __restrict__ fent_d,
+                                                        const T* __restrict__ fent_v,
+                                                        const uint32_t* __restrict__ fcur, uint32_t NBf,
+                                                        uint32_t bsf, uint32_t G, const T* __restrict__ src,
+                                                        uint32_t src_cap, uint32_t sZ) {
   const uint32_t w = blockIdx.x;
   const uint32_t x = w & 7u, rest = w >> 3;
   const uint32_t g = rest % G, bin = (rest / G) * 8u + x;
-  const uint32_t s = binbase[bin], e = binbase[bin + 1];
-  const uint32_t per = (e - s + G - 1) / G;
-  const uint32_t lo = s + g * per, hi = min(lo + per, e);
+  if (bin >= NBf) return;
+  const uint32_t cnt = min(fcur[bin], 1u << bsf);
+  const size_t s = (size_t)bin << bsf;
+  const uint32_t per = (cnt + G - 1) / G;
+  const uint32_t lo = g * per, hi = min(lo + per, cnt);
   for (uint32_t q = lo + threadIdx.x; q < hi; q += 1024u) {
-    uint32_t d = ent_d[q];
-    T v = ent_v[q];
+    uint32_t d = fent_d[s + q];
+    T v = fent_v[s + q];
     if (d & kDenseOrphan) {
       const uint32_t t = value_rank<T>(v);
       if (t >= src_cap) continue;
@@ -282,78 +439,240 @@ __global__ __launch_bounds__(1024) void rz_dense_scatter(T* __restrict__ A, cons
   }
 }
 
+// ---- bucketed tail marks ------------------------------------------------------
+template <int WG, int IPT>
+__global__ __launch_bounds__(WG) void rz_dense_tail_count(const int32_t* __restrict__ R, uint32_t k, uint32_t sZ,
+                                                           uint32_t L, uint32_t NBt, uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t hist[kCoarseMax];
+  const uint32_t tid = threadIdx.x, j = blockIdx.x;
+  const uint32_t a = j * L, b = min(a + L, k);
+  for (uint32_t q = tid; q < NBt; q += WG) hist[q] = 0u;
+  __syncthreads();
+  for (uint32_t base = a; base < b; base += WG * IPT) {
+    uint32_t z[IPT];
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+      const uint32_t p = base + q * WG + tid;
+      z[q] = p < b ? (uint32_t)R[p] - sZ : 0xffffffffu;  // wraps to >= k below the red zone
+    }
+#pragma unroll
+    for (int q = 0; q < IPT; ++q)
+      if (z[q] < k) atomicAdd(&hist[z[q] >> kTailBinLog], 1u);
+  }
+  __syncthreads();
+  for (uint32_t q = tid; q < NBt; q += WG) cnt[(size_t)j * NBt + q] = hist[q];
+}
+
+// Dynamic LDS: S red-zone offsets and S bin ids (S = WG * IPT).
+template <int WG, int IPT>
+__global__ __launch_bounds__(WG) void rz_dense_tail_entries(const int32_t* __restrict__ R, uint32_t k, uint32_t sZ,
+                                                             uint32_t L, uint32_t NBt,
+                                                             const uint32_t* __restrict__ off,
+                                                             const uint32_t* __restrict__ binbase,
+                                                             uint32_t* __restrict__ ent) {
+  constexpr uint32_t S = WG * IPT;
+  extern __shared__ __align__(16) unsigned char dyn[];
+  uint32_t* sz = (uint32_t*)dyn;
+  uint16_t* sb = (uint16_t*)(sz + S);
+  __shared__ uint32_t lcnt[kCoarseMax + 1], gb[kCoarseMax], gcur[kCoarseMax];
+  __shared__ uint32_t tmp[WG / kWave + 1];
+  const uint32_t tid = threadIdx.x, j = blockIdx.x;
+  const uint32_t a = j * L, b = min(a + L, k);
+  for (uint32_t q = tid; q < NBt; q += WG) {
+    gcur[q] = binbase[q] + off[(size_t)j * NBt + q];
+    lcnt[q] = 0u;
+  }
+  __syncthreads();
+  for (uint32_t base = a; base < b; base += S) {
+    uint32_t z[IPT], rk[IPT];
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+      const uint32_t p = base + q * WG + tid;
+      z[q] = p < b ? (uint32_t)R[p] - sZ : 0xffffffffu;
+    }
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) rk[q] = z[q] < k ? atomicAdd(&lcnt[z[q] >> kTailBinLog], 1u) : 0u;
+    __syncthreads();
+    for (uint32_t q = tid; q < NBt; q += WG) {
+      gb[q] = gcur[q];
+      gcur[q] += lcnt[q];
+    }
+    __syncthreads();
+    const uint32_t total = lds_excl_scan<WG>(lcnt, NBt, tmp);
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+      if (z[q] >= k) continue;
+      const uint32_t bin = z[q] >> kTailBinLog;
+      const uint32_t e = lcnt[bin] + rk[q];
+      sz[e] = z[q];
+      sb[e] = (uint16_t)bin;
+    }
+    __syncthreads();
+    for (uint32_t e = tid; e < total; e += WG) {
+      const uint32_t bin = sb[e];
+      ent[gb[bin] + (e - lcnt[bin])] = sz[e];
+    }
+    __syncthreads();
+    for (uint32_t q = tid; q < NBt; q += WG) lcnt[q] = 0u;
+    __syncthreads();
+  }
+}
+
+// One workgroup per tail bin: set the bin's bits in a dynamic-LDS bitmap of
+// 2^kTailBinLog bits, then write its words whole (every word of the k-bit
+// bitmap belongs to exactly one bin).
+__global__ __launch_bounds__(1024) void rz_dense_tail_mark(const uint32_t* __restrict__ ent,
+                                                          const uint32_t* __restrict__ binbase, uint32_t k,
+                                                          uint32_t* __restrict__ bits,
+                                                          uint64_t* __restrict__ stat_tail, uint32_t NBt) {
+  constexpr uint32_t W = 1u << (kTailBinLog - 5);
+  extern __shared__ __align__(16) unsigned char dyn[];
+  uint32_t* lb = (uint32_t*)dyn;
+  const uint32_t bin = blockIdx.x, tid = threadIdx.x;
+  for (uint32_t w = tid; w < W; w += 1024u) lb[w] = 0u;
+  __syncthreads();
+  const uint32_t z0 = bin << kTailBinLog;
+  const uint32_t s = binbase[bin], e = binbase[bin + 1];
+  for (uint32_t q = s + tid; q < e; q += 1024u) {
+    const uint32_t r = ent[q] - z0;  // < 2^kTailBinLog for an entry of this bin
+    if (r < (1u << kTailBinLog)) atomicOr(&lb[r >> 5], 1u << (r & 31));
+  }
+  __syncthreads();
+  const uint32_t nw = (k + 31u) >> 5;
+  for (uint32_t w = tid; w < W; w += 1024u) {
+    const uint32_t gw = (z0 >> 5) + w;
+    if (gw < nw) bits[gw] = lb[w];
+  }
+  if (stat_tail && bin == 0 && tid == 0) *stat_tail = binbase[NBt];
+}
+
 // ---------------------------------------------------------------------------
 // launch
 // ---------------------------------------------------------------------------
 struct DenseShape {
-  uint32_t L, nt, NB, bs, G;
+  uint32_t L, nt;     // tiles of the removal list
+  uint32_t NB, bs;    // coarse destination bins (multiple of 32)
+  uint32_t NBf, bsf;  // fine destination bins (multiple of 8)
+  uint32_t NBt;       // tail bins (multiple of 32), 0 = atomic marks
+  uint32_t G;         // scatter workgroups per fine bin
 };
 
-// Bins: NB (a multiple of 32, at most 1024) bins of 2^bs positions covering
-// [0, sZ); 2^slice_steps is the target bin count.  Tiles: as the move slices.
-static bool dense_shape(int64_t n, int64_t k, const rz_tile_config& cfg, DenseShape* sh) {
-  const int64_t sZ = n - k;
-  int lg = cfg.slice_steps > 0 ? cfg.slice_steps : 9;
-  if (lg > 10) lg = 10;
-  if (lg < 5) lg = 5;
-  uint32_t bs = 10;
-  while (((sZ + ((int64_t)1 << bs) - 1) >> bs) > ((int64_t)1 << lg)) ++bs;
-  uint32_t nb = (uint32_t)((sZ + ((int64_t)1 << bs) - 1) >> bs);
-  nb = (nb + 31u) & ~31u;
-  if (nb < 32) nb = 32;
-  if (nb > (uint32_t)kDenseMaxBins) return false;
-  uint32_t L, nt;
-  slice_shape(k, cfg, kMaxSlices, &L, &nt);
-  uint32_t G = 2048u / nb;
-  if (G < 1) G = 1;
-  *sh = DenseShape{L, nt, nb, bs, G};
-  return true;
+static uint32_t bin_log(int64_t span, int64_t target) {
+  uint32_t lg = 10;
+  while (((span + ((int64_t)1 << lg) - 1) >> lg) > target) ++lg;
+  return lg;
 }
 
+// small buffer: [btot | binbase (+1) | tbtot | tbinbase (+1) | fcur] (uint32,
+// kSmallStride each, fcur kFineMax + 64), then [ocnt | obase (+1)] (uint64)
+constexpr size_t kSmallStride = kCoarseMax + 64;
+constexpr size_t kSmallWords = 4 * kSmallStride + kFineMax + 64;
+constexpr size_t kSmallBytes = kSmallWords * 4 + 2 * ((size_t)kMaxSlices + 64) * 8;
+
 template <typename T, int WG, int IPT>
-static void dense_launch(rz_ctx* ctx, T* A, const int32_t* R, uint32_t k, uint32_t sZ, const DenseShape& d,
-                         uint32_t* bits, uint32_t* ctrl, int stats, uint32_t src_cap, hipStream_t s) {
+static rz_status dense_launch(rz_ctx* ctx, T* A, const int32_t* R, uint32_t k, uint32_t sZ, const DenseShape& d,
+                              uint32_t* bits, uint32_t* ctrl, int stats, uint32_t src_cap, hipStream_t s) {
+  constexpr uint32_t S = WG * IPT;
+  const size_t lds_ent = (size_t)S * (sizeof(T) + 4 + 2);
+  const size_t lds_tail = (size_t)S * (4 + 2);
+  const size_t lds_mark = (size_t)4 << (kTailBinLog - 5);
+  static bool attr_set = false;
+  if (!attr_set) {
+    RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_entries<T, WG, IPT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_ent));
+    RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_refine<T, WG, IPT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_ent));
+    RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_tail_entries<WG, IPT>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_tail));
+    RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_tail_mark, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_mark));
+    attr_set = true;
+  }
   uint32_t* cnt = (uint32_t*)ctx->dense_cnt.ptr;
-  uint32_t* off = cnt + (size_t)d.nt * d.NB;
   uint32_t* btot = (uint32_t*)ctx->dense_small.ptr;
-  uint32_t* binbase = btot + kDenseMaxBins;
-  uint64_t* ocnt = (uint64_t*)(binbase + kDenseMaxBins + 64);
-  uint64_t* obase = ocnt + kMaxSlices;
+  uint32_t* binbase = btot + kSmallStride;
+  uint32_t* tbtot = binbase + kSmallStride;
+  uint32_t* tbinbase = tbtot + kSmallStride;
+  uint32_t* fcur = tbinbase + kSmallStride;
+  uint64_t* ocnt = (uint64_t*)((uint32_t*)ctx->dense_small.ptr + kSmallWords);
+  uint64_t* obase = ocnt + kMaxSlices + 64;
   uint32_t* ent_d = (uint32_t*)ctx->dense_ent.ptr;
   T* ent_v = (T*)((char*)ctx->dense_ent.ptr + (((size_t)k * 4 + 255) & ~(size_t)255));
+  const size_t fcap = (size_t)d.NBf << d.bsf;
+  uint32_t* fent_d = (uint32_t*)ctx->dense_fine.ptr;
+  T* fent_v = (T*)((char*)ctx->dense_fine.ptr + ((fcap * 4 + 255) & ~(size_t)255));
   T* src = (T*)ctx->dense_src.ptr;
+  if (d.NBt) {
+    // the red-zone offsets use ent_d's space before the head writes do
+    uint32_t* toff = cnt + (size_t)d.nt * d.NBt;
+    hipLaunchKernelGGL((rz_dense_tail_count<WG, IPT>), dim3(d.nt), dim3(WG), 0, s, R, k, sZ, d.L, d.NBt, cnt);
+    hipLaunchKernelGGL(rz_dense_colscan, dim3(d.NBt / 32), dim3(1024), 0, s, cnt, d.nt, d.NBt, toff, tbtot);
+    hipLaunchKernelGGL(rz_dense_topscan, dim3(1), dim3(1024), 0, s, tbtot, d.NBt, tbinbase, ocnt, 0u,
+                       (uint64_t*)nullptr, ctrl, 0);
+    hipLaunchKernelGGL((rz_dense_tail_entries<WG, IPT>), dim3(d.nt), dim3(WG), lds_tail, s, R, k, sZ, d.L, d.NBt,
+                       toff, tbinbase, ent_d);
+    hipLaunchKernelGGL(rz_dense_tail_mark, dim3(d.NBt), dim3(1024), lds_mark, s, ent_d, tbinbase, k, bits,
+                       stats ? (uint64_t*)(ctrl + kCtrlStatTail) : (uint64_t*)nullptr, d.NBt);
+  }
+  uint32_t* off = cnt + (size_t)d.nt * d.NB;
   hipLaunchKernelGGL((rz_dense_count<WG, IPT>), dim3(d.nt), dim3(WG), 0, s, R, k, sZ, d.L, bits, d.NB, d.bs, cnt,
-                     ocnt);
+                     ocnt, fcur, d.NBf);
   hipLaunchKernelGGL(rz_dense_colscan, dim3(d.NB / 32), dim3(1024), 0, s, cnt, d.nt, d.NB, off, btot);
   hipLaunchKernelGGL(rz_dense_topscan, dim3(1), dim3(1024), 0, s, btot, d.NB, binbase, ocnt, d.nt, obase, ctrl,
                      stats);
-  hipLaunchKernelGGL((rz_dense_entries<T, WG, IPT>), dim3(d.nt), dim3(WG), 0, s, A, R, k, sZ, d.L, bits, d.NB, d.bs,
-                     off, binbase, obase, ent_d, ent_v, src, src_cap);
-  hipLaunchKernelGGL(rz_dense_scatter<T>, dim3(d.NB * d.G), dim3(1024), 0, s, A, ent_d, ent_v, binbase, d.G, src,
-                     src_cap, sZ);
+  hipLaunchKernelGGL((rz_dense_entries<T, WG, IPT>), dim3(d.nt), dim3(WG), lds_ent, s, A, R, k, sZ, d.L, bits, d.NB,
+                     d.bs, off, binbase, obase, ent_d, ent_v, src, src_cap);
+  hipLaunchKernelGGL((rz_dense_refine<T, WG, IPT>), dim3((k + S - 1) / S), dim3(WG), lds_ent, s, ent_d, ent_v,
+                     binbase + d.NB, fent_d, fent_v, fcur, d.NBf, d.bsf, ctrl);
+  hipLaunchKernelGGL(rz_dense_scatter<T>, dim3(d.NBf * d.G), dim3(1024), 0, s, A, fent_d, fent_v, fcur, d.NBf, d.bsf,
+                     d.G, src, src_cap, sZ);
+  RZ_HIP(hipGetLastError());
+  return RZ_OK;
+}
+
+bool dense_bucket_marks(const rz_tile_config& cfg, int64_t k, uint32_t flags) {
+  return cfg.variant == 5 && cfg.chunk == 2 && k <= ((int64_t)kCoarseMax << kTailBinLog) && !(flags & RZ_SORTED);
 }
 
 template <typename T>
 rz_status dense_moves(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, int64_t k, const rz_tile_config& cfg,
-                      uint32_t* bits, uint32_t* ctrl, int stats, hipStream_t s) {
+                      uint32_t* bits, uint32_t* ctrl, int stats, bool bucket_marks, hipStream_t s) {
+  const int64_t sZ = n - k;
   DenseShape d;
-  if (!dense_shape(n, k, cfg, &d)) return RZ_EINVAL;
+  slice_shape(k, cfg, kMaxSlices, &d.L, &d.nt);
+  d.bs = bin_log(sZ, kCoarseTarget);
+  d.NB = (uint32_t)((sZ + ((int64_t)1 << d.bs) - 1) >> d.bs);
+  d.NB = d.NB < 32 ? 32 : (d.NB + 31u) & ~31u;
+  int lg = cfg.slice_steps > 0 ? cfg.slice_steps : 10;
+  if (lg > 12) lg = 12;
+  if (lg < 3) lg = 3;
+  d.bsf = bin_log(sZ, (int64_t)1 << lg);
+  d.NBf = (uint32_t)((sZ + ((int64_t)1 << d.bsf) - 1) >> d.bsf);
+  d.NBf = (d.NBf + 7u) & ~7u;
+  d.NBt = 0;
+  if (bucket_marks) {
+    d.NBt = (uint32_t)((k + ((int64_t)1 << kTailBinLog) - 1) >> kTailBinLog);
+    d.NBt = (d.NBt + 31u) & ~31u;
+  }
+  // scatter: every XCD holds about two 1024-thread workgroups per CU, so
+  // G = CUs per XCD keeps about two fine bins of each XCD in flight
+  d.G = (uint32_t)(ctx->cu_count / 8 > 0 ? ctx->cu_count / 8 : 1);
+  if (d.NB > (uint32_t)kCoarseMax || d.NBf > (uint32_t)kFineMax || d.NBt > (uint32_t)kCoarseMax) return RZ_EINVAL;
   const uint32_t src_cap = (uint32_t)(k / 2 + 1);
+  const uint32_t nbmax = d.NB > d.NBt ? d.NB : d.NBt;
+  const size_t fcap = (size_t)d.NBf << d.bsf;
   rz_status st;
-  if ((st = ensure(ctx, ctx->dense_cnt, (size_t)d.nt * d.NB * 4 * 2, s)) != RZ_OK) return st;
-  if ((st = ensure(ctx, ctx->dense_small, (size_t)(2 * kDenseMaxBins + 64) * 4 + (size_t)2 * (kMaxSlices + 1) * 8,
-                   s)) != RZ_OK)
-    return st;
+  if ((st = ensure(ctx, ctx->dense_cnt, (size_t)d.nt * nbmax * 4 * 2, s)) != RZ_OK) return st;
+  if ((st = ensure(ctx, ctx->dense_small, kSmallBytes, s)) != RZ_OK) return st;
   if ((st = ensure(ctx, ctx->dense_ent, (((size_t)k * 4 + 255) & ~(size_t)255) + (size_t)k * sizeof(T), s)) != RZ_OK)
     return st;
+  if ((st = ensure(ctx, ctx->dense_fine, ((fcap * 4 + 255) & ~(size_t)255) + fcap * sizeof(T), s)) != RZ_OK)
+    return st;
   if ((st = ensure(ctx, ctx->dense_src, (size_t)src_cap * sizeof(T), s)) != RZ_OK) return st;
-  const uint32_t sZ = (uint32_t)(n - k);
-#define RZ_DENSE_CASE(W, I)                                                                             \
-  if (cfg.wg_size == W && cfg.items == I) {                                                             \
-    dense_launch<T, W, I>(ctx, A, R, (uint32_t)k, sZ, d, bits, ctrl, stats, src_cap, s);                \
-    RZ_HIP(hipGetLastError());                                                                          \
-    return RZ_OK;                                                                                       \
-  }
+  const size_t lds = (size_t)cfg.wg_size * cfg.items * (sizeof(T) + 6) + 48 * 1024;  // + static LDS (refine)
+  if (lds > (size_t)ctx->lds_max) return RZ_EINVAL;
+#define RZ_DENSE_CASE(W, I) \
+  if (cfg.wg_size == W && cfg.items == I) return dense_launch<T, W, I>(ctx, A, R, (uint32_t)k, (uint32_t)sZ, d, bits, ctrl, stats, src_cap, s);
   RZ_DENSE_CASE(256, 4) RZ_DENSE_CASE(256, 8) RZ_DENSE_CASE(512, 4) RZ_DENSE_CASE(512, 8)
   RZ_DENSE_CASE(1024, 4) RZ_DENSE_CASE(1024, 8)
 #undef RZ_DENSE_CASE
@@ -361,10 +680,10 @@ rz_status dense_moves(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, int64_t k,
 }
 
 template rz_status dense_moves<uint32_t>(rz_ctx*, uint32_t*, int64_t, const int32_t*, int64_t,
-                                         const rz_tile_config&, uint32_t*, uint32_t*, int, hipStream_t);
+                                         const rz_tile_config&, uint32_t*, uint32_t*, int, bool, hipStream_t);
 template rz_status dense_moves<uint64_t>(rz_ctx*, uint64_t*, int64_t, const int32_t*, int64_t,
-                                         const rz_tile_config&, uint32_t*, uint32_t*, int, hipStream_t);
+                                         const rz_tile_config&, uint32_t*, uint32_t*, int, bool, hipStream_t);
 template rz_status dense_moves<uint4>(rz_ctx*, uint4*, int64_t, const int32_t*, int64_t, const rz_tile_config&,
-                                      uint32_t*, uint32_t*, int, hipStream_t);
+                                      uint32_t*, uint32_t*, int, bool, hipStream_t);
 
 }  // namespace rz

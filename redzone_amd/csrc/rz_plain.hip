@@ -511,9 +511,12 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
     if ((st = ensure(ctx, ctx->slots, ((size_t)k / 2 + 1) * 8, s)) != RZ_OK) return st;
     epoch = ctrl + kCtrlMoveEpoch;
   }
+  const bool dense_marks = dense_bucket_marks(cfg, k, flags);
   const int64_t g1w = (k + 256 * 4 - 1) / (256 * 4);
   const dim3 g1((uint32_t)(g1w < (int64_t)ctx->cu_count * 8 ? g1w : ctx->cu_count * 8));
-  if (stats)
+  if (dense_marks) {
+    // the dense form sets the tail marks itself
+  } else if (stats)
     hipLaunchKernelGGL(rz_mark_tail<true>, g1, dim3(256), 0, s, R, (uint32_t)k, sZ, bits,
                        (uint64_t*)(ctrl + kCtrlStatTail), epoch);
   else if (flags & RZ_SORTED)
@@ -525,7 +528,7 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
   RZ_HIP(hipGetLastError());
   if ((st = prof_mark(ctx, flags, 1, s)) != RZ_OK) return st;
   if (cfg.variant == 5) {
-    if ((st = dense_moves<T>(ctx, A, n, R, k, cfg, bits, ctrl, stats, s)) != RZ_OK) return st;
+    if ((st = dense_moves<T>(ctx, A, n, R, k, cfg, bits, ctrl, stats, dense_marks, s)) != RZ_OK) return st;
     if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;
     return prof_mark(ctx, flags, 3, s);
   }

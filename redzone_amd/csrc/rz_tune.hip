@@ -51,13 +51,13 @@ static float median(std::vector<float> v) {
 
 static const rz_tile_config kCandidates[] = {
     // wg, items, max_slices, variant, slice_steps, chunk
-    {256, 4, 2048, 0, 0, 1},  {256, 8, 4096, 0, 0, 1},  {512, 8, 4096, 0, 0, 1},  {512, 16, 4096, 0, 0, 1},
-    {256, 4, 512, 0, 0, 1},   {512, 8, 1024, 0, 0, 1},  {256, 8, 4096, 3, 0, 1},  {512, 8, 4096, 3, 0, 1},
-    {256, 4, 4096, 3, 0, 1},  {256, 8, 0, 2, 1, 1},     {512, 8, 0, 2, 1, 1},     {256, 4, 0, 2, 2, 1},
-    {256, 4, 512, 4, 0, 1},   {256, 4, 2048, 4, 0, 1},  {256, 8, 4096, 4, 0, 1},  {512, 8, 1024, 4, 0, 1},
-    {512, 8, 4096, 4, 0, 1},  {256, 2, 4096, 4, 0, 1},
-    {512, 8, 1024, 5, 9, 1},  {512, 8, 2048, 5, 10, 1}, {1024, 8, 512, 5, 9, 1}, {1024, 4, 4096, 5, 10, 1},
-    {256, 8, 4096, 5, 8, 1},
+    {256, 4, 512, 0, 0, 1},   {512, 8, 4096, 0, 0, 1},  {256, 8, 4096, 0, 0, 1},                            // three kernels
+    {256, 4, 4096, 3, 0, 1},  {256, 8, 4096, 3, 0, 1},  {512, 8, 4096, 3, 0, 1},                            // ordered
+    {256, 8, 0, 2, 1, 1},                                                                                   // queue
+    {256, 4, 512, 4, 0, 1},   {256, 4, 2048, 4, 0, 1},  {256, 8, 512, 4, 0, 1},  {512, 8, 4096, 4, 0, 1},   // fused
+    {256, 2, 4096, 4, 0, 1},  {256, 8, 4096, 4, 0, 1},
+    {1024, 8, 1024, 5, 11, 2}, {1024, 8, 1024, 5, 12, 2}, {1024, 4, 1024, 5, 11, 2}, {512, 8, 2048, 5, 11, 2},  // dense
+    {1024, 8, 1024, 5, 11, 0},
 };
 
 static const rz_tile_config kBatchCandidates[] = {

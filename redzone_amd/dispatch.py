@@ -11,9 +11,10 @@ kernels; this module only drives them (collects timings, batches the
 training, stores the parameters).
 
 Methods (outputs of the network):
-  0  redzone         rz_remove with the tuned launch shape (three kernels)
+  0  redzone         rz_remove with the tuned launch shape for (element size, log2 k)
   1  redzone_single  one ordered launch (roles by workgroup id)
-  2  redzone_queue   the persistent work-queue kernel with one list
+  2  redzone_dense   the region-bucketed form (variant 5), which pays off when
+                     k/n is large enough that holes share DRAM bursts
   3  partition       rz_partition_on (mark + stable O(n) compaction)
 """
 import ctypes
@@ -31,9 +32,18 @@ NC = 4
 NP = NH * NF + NH + 2 * (NH * NH + NH) + NC * NH + NC
 assert NP == 2660
 
-METHODS = ["redzone", "redzone_single", "redzone_queue", "partition"]
-METHOD_CONFIGS = [None, TileConfig(256, 4, 4096, 3, 0, 1), TileConfig(256, 8, 0, 2, 1, 1), None]
+METHODS = ["redzone", "redzone_single", "redzone_dense", "partition"]
 RULE_REDZONE, RULE_PARTITION = 0, 3
+
+
+def method_config(method, elem_bytes):
+    """Launch shape of a Redzone method (None: the tuned table)."""
+    if method == 1:
+        return TileConfig(256, 4, 4096, 3, 0, 1)
+    if method == 2:
+        # 16-byte items stage half as many entries per step (LDS)
+        return TileConfig(1024, 4 if elem_bytes == 16 else 8, 1024, 5, 11, 2)
+    return None
 
 MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dispatch_model_gfx950.npz")
 
@@ -47,7 +57,9 @@ def run_method(ctx, method, data, removals, stream=None):
     """Runs method `method` on one list (in place); returns n - k."""
     if method == 3:
         return ctx.partition(data, removals, stream=stream)
-    return ctx.remove(data, removals, config=METHOD_CONFIGS[method], stream=stream)
+    from . import elem_bytes_of
+
+    return ctx.remove(data, removals, config=method_config(method, elem_bytes_of(data)), stream=stream)
 
 
 def init_params(seed=0):

@@ -184,7 +184,7 @@ def evaluate(ctx, disp, specs, seed, reps=3, budget=8 << 30, log=None):
     """Mixed-stream evaluation; returns a dict of totals and regrets."""
     import torch
 
-    from .dispatch import rule_choice
+    from .dispatch import METHODS, rule_choice
 
     tot = {"plain_rule_wall_s": 0.0, "batched_learned_wall_s": 0.0, "oracle_best_us": 0.0, "learned_us": 0.0,
            "rule_us": 0.0, "redzone_only_us": 0.0}
@@ -227,7 +227,7 @@ def evaluate(ctx, disp, specs, seed, reps=3, budget=8 << 30, log=None):
     tot["learned_regret"] = tot["learned_us"] / tot["oracle_best_us"] - 1
     tot["rule_regret"] = tot["rule_us"] / tot["oracle_best_us"] - 1
     tot["batched_learned_vs_plain"] = tot["plain_rule_wall_s"] / tot["batched_learned_wall_s"]
-    tot["picks"] = {m: int(c) for m, c in zip(["redzone", "redzone_single", "redzone_queue", "partition"], picks)}
+    tot["picks"] = {m: int(c) for m, c in zip(METHODS, picks)}
     tot["_per_list_us"] = per_list
     tot["_chosen"] = chosen
     return tot

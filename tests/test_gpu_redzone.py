@@ -168,8 +168,8 @@ def test_dense_variant(ctx, oracle, esz):
         A = _payload(esz, n, seed=39)
         R = W.removals(pattern, n, k, seed=40)
         want = expected_canonical(oracle, A.numpy(), R.numpy())
-        for wg, it in ((256, 4), (512, 8), (1024, 4)):
-            cfg = rz.TileConfig(wg, it, ms, 5, lgb, 0)
+        for wg, it, marks in ((256, 4, 0), (512, 8, 2), (1024, 4, 2)):
+            cfg = rz.TileConfig(wg, it, ms, 5, lgb, marks)
             announce("dense", esz, n, k, pattern, cfg)
             gpu_remove_parity(ctx, oracle, A, R, config=cfg, want=want)
 
@@ -179,7 +179,7 @@ def test_dense_variant_flags(ctx, oracle):
     k = n // 4
     A = W.index_payload(n)
     R = W.uniform_removals(n, k, seed=41)
-    cfg = rz.TileConfig(512, 8, 1024, 5, 9, 0)
+    cfg = rz.TileConfig(512, 8, 1024, 5, 9, 2)
     announce("dense stats")
     _, dR = gpu_remove_parity(ctx, oracle, A, R, config=cfg, flags=rz.RZ_STATS | rz.RZ_PRESERVE_REMOVALS)
     s = ctx.last_stats()
