@@ -150,18 +150,18 @@ def run_plain(ctx, lists, specs):
     return time.perf_counter() - t0
 
 
-# Lists with at least this many removals are launched on their own: their
-# time is their random writes, which the three-kernel launch handles best, while
-# the persistent queue pays off for the many small lists whose time is launch
-# overhead (a 4096-list batch costs about half a microsecond per list).
-BATCH_MAX_K = 1 << 17
+# Lists with fewer removals than this go into one persistent queue launch,
+# whatever method the network prefers for them on their own: a list's own
+# launch costs about 10 us, a list inside a batch about half a microsecond, and
+# the queue moves large lists at the same random-write rate as their own
+# launch.  Larger lists are launched on their own with the chosen method.
+BATCH_MAX_K = 1 << 20
 
 
-def run_batched_learned(ctx, disp, lists, specs):
-    """Features + MLP choice on the device; small Redzone lists in one
-    persistent launch, large ones with their chosen Redzone form, partition
-    lists one launch each; wall time (s) of everything including the decision
-    and the batch plan."""
+def run_batched_learned(ctx, disp, lists, specs, batch_max_k=BATCH_MAX_K):
+    """Features + MLP choice on the device; lists below batch_max_k removals in
+    one persistent launch, the others with their chosen method; wall time (s)
+    of everything including the decision and the batch plan."""
     import torch
 
     from .dispatch import run_method
@@ -170,24 +170,26 @@ def run_batched_learned(ctx, disp, lists, specs):
     t0 = time.perf_counter()
     X = disp.features(lists)
     choice = disp.choose(X)
-    small = [i for i in range(len(lists)) if choice[i] != 3 and specs[i][1] < BATCH_MAX_K]
+    small = [i for i in range(len(lists)) if specs[i][1] < batch_max_k]
     if small:
         ctx.remove_batched([lists[i] for i in small])
     for i in range(len(lists)):
-        if choice[i] == 3 or specs[i][1] >= BATCH_MAX_K:
+        if specs[i][1] >= batch_max_k:
             run_method(ctx, int(choice[i]), lists[i][0], lists[i][1])
     torch.cuda.synchronize()
     return time.perf_counter() - t0, choice
 
 
-def evaluate(ctx, disp, specs, seed, reps=3, budget=8 << 30, log=None):
-    """Mixed-stream evaluation; returns a dict of totals and regrets."""
+def evaluate(ctx, disp, specs, seed, reps=3, budget=8 << 30, log=None, sweep=(1 << 17, 1 << 22)):
+    """Mixed-stream evaluation; returns a dict of totals and regrets.  `sweep`
+    lists further batch thresholds whose batched wall time is also measured."""
     import torch
 
     from .dispatch import METHODS, rule_choice
 
     tot = {"plain_rule_wall_s": 0.0, "batched_learned_wall_s": 0.0, "oracle_best_us": 0.0, "learned_us": 0.0,
            "rule_us": 0.0, "redzone_only_us": 0.0}
+    swept = {int(b): 0.0 for b in sweep}
     picks = np.zeros(4, np.int64)
     per_list = np.zeros((len(specs), 4), np.float64)
     chosen = np.zeros(len(specs), np.int32)
@@ -220,6 +222,9 @@ def evaluate(ctx, disp, specs, seed, reps=3, budget=8 << 30, log=None):
         restore()
         wall, _ = run_batched_learned(ctx, disp, work, sub)
         tot["batched_learned_wall_s"] += wall
+        for b in swept:
+            restore()
+            swept[b] += run_batched_learned(ctx, disp, work, sub, batch_max_k=b)[0]
         del pristine, work
         torch.cuda.empty_cache()
         if log:
@@ -227,6 +232,8 @@ def evaluate(ctx, disp, specs, seed, reps=3, budget=8 << 30, log=None):
     tot["learned_regret"] = tot["learned_us"] / tot["oracle_best_us"] - 1
     tot["rule_regret"] = tot["rule_us"] / tot["oracle_best_us"] - 1
     tot["batched_learned_vs_plain"] = tot["plain_rule_wall_s"] / tot["batched_learned_wall_s"]
+    tot["batch_max_k"] = BATCH_MAX_K
+    tot["batched_wall_s_by_batch_max_k"] = {str(b): w for b, w in swept.items()}
     tot["picks"] = {m: int(c) for m, c in zip(METHODS, picks)}
     tot["_per_list_us"] = per_list
     tot["_chosen"] = chosen
