@@ -17,7 +17,7 @@ from . import _native
 from ._native import (RZ_COOPERATIVE, RZ_PRESERVE_REMOVALS, RZ_PROFILE, RZ_SORTED, RZ_STATS, RZ_VALIDATE, BatchInfo, ListDesc,
                       RedzoneError, Stats, TileConfig)
 
-__all__ = ["Context", "Batch", "remove", "partition", "TileConfig", "Stats", "RedzoneError", "RZ_PRESERVE_REMOVALS",
+__all__ = ["Context", "Batch", "ListDescs", "remove", "partition", "TileConfig", "Stats", "RedzoneError", "RZ_PRESERVE_REMOVALS",
            "RZ_VALIDATE", "RZ_COOPERATIVE", "RZ_STATS", "RZ_PROFILE", "RZ_SORTED", "elem_bytes_of", "default_context"]
 
 
@@ -219,10 +219,64 @@ class Context:
         self._call("rz_batch_create", self._h, arr, cnt, flags, cfg, ctypes.byref(h))
         return Batch(self, h)
 
+    def descs(self, lists):
+        """ListDescs (the C ABI's rz_list_desc rows) of [(data, removals), ...],
+        built once so that batches, feature extraction and per-list launches can
+        share it."""
+        import numpy as np
+        import torch
+
+        i32 = torch.int32
+        rows = []
+        for data, rem in lists:
+            if not (data.is_cuda and rem.is_cuda) or rem.dtype is not i32:
+                raise ValueError("lists must hold device tensors and int32 removals")
+            if data.get_device() != self.device or rem.get_device() != self.device:
+                raise ValueError(f"tensors must be on cuda:{self.device}")
+            if not (data.is_contiguous() and rem.is_contiguous()):
+                raise ValueError("tensors must be contiguous")
+            rows.append((data.data_ptr(), rem.data_ptr(), data.shape[0], rem.shape[0], elem_bytes_of(data), 0))
+        return ListDescs(np.array(rows, dtype=LIST_DESC_DTYPE) if rows else np.zeros(0, LIST_DESC_DTYPE))
+
     def remove_batched(self, lists, flags=0, stream=None):
-        """One-shot batched removal over [(data, removals), ...]."""
-        arr, cnt = self._descs(lists)
+        """One-shot batched removal over [(data, removals), ...] or a ListDescs."""
+        if isinstance(lists, ListDescs):
+            arr, cnt = lists.as_ctypes(), len(lists)
+        else:
+            arr, cnt = self._descs(lists)
         self._call("rz_remove_batched", self._h, arr, cnt, flags, _stream_handle(stream, self.device))
+
+
+# rz_list_desc as a numpy record (40 bytes, the C layout)
+def _list_desc_dtype():
+    import numpy as np
+
+    return np.dtype([("data", "<u8"), ("removals", "<u8"), ("n", "<i8"), ("k", "<i8"), ("elem_bytes", "<u4"),
+                     ("method", "<u4")])
+
+
+LIST_DESC_DTYPE = _list_desc_dtype()
+assert LIST_DESC_DTYPE.itemsize == ctypes.sizeof(ListDesc)
+
+
+class ListDescs:
+    """Host array of rz_list_desc rows (see Context.descs); subset(idx) selects rows."""
+
+    def __init__(self, rows):
+        import numpy as np
+
+        self.rows = np.ascontiguousarray(rows, dtype=LIST_DESC_DTYPE)
+
+    def __len__(self):
+        return len(self.rows)
+
+    def subset(self, idx):
+        import numpy as np
+
+        return ListDescs(self.rows[np.asarray(idx, np.int64)])
+
+    def as_ctypes(self):
+        return self.rows.ctypes.data_as(ctypes.POINTER(ListDesc))
 
 
 class Batch:

@@ -94,13 +94,15 @@ class Dispatcher:
 
     # ---- features -------------------------------------------------------
     def features(self, lists, stream=None):
-        """lists: [(data, removals), ...] device tensors -> float32 device tensor [B, NF]."""
+        """lists: [(data, removals), ...] device tensors, or a ListDescs -> float32 device tensor [B, NF]."""
         import torch
 
-        descs, cnt = self.ctx._descs(lists)
-        raw = torch.frombuffer(bytearray(ctypes.string_at(ctypes.addressof(descs), ctypes.sizeof(ListDesc) * cnt)),
-                               dtype=torch.uint8)
-        d_descs = raw.cuda(self.ctx.device)
+        from . import ListDescs
+
+        if not isinstance(lists, ListDescs):
+            lists = self.ctx.descs(lists)
+        cnt = len(lists)
+        d_descs = torch.from_numpy(lists.rows.view(np.uint8)).cuda(self.ctx.device)
         out = torch.empty((cnt, NF), dtype=torch.float32, device=f"cuda:{self.ctx.device}")
         self.ctx._call("rz_features", self.ctx.handle, _ptr(d_descs), cnt, _ptr(out),
                        _stream_ptr(stream, self.ctx.device))

@@ -14,6 +14,7 @@ patterns) and processed in resident batches.
     python -m redzone_amd.mixed eval  [--lists 10000] [--seed 2]  # prints one JSON line
 """
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -136,16 +137,57 @@ def train_model(ctx, feats, times, epochs=600, lr=3e-3, seed=0):
     return disp
 
 
-def run_plain(ctx, lists, specs):
-    """Per-list launches, method by the k < n/2 rule; wall time (s) from first launch to the end."""
+def _launch_each(ctx, rows, methods, stream):
+    """One C-ABI launch per list (rows: rz_list_desc tuples), as a native host
+    would issue them: rz_remove / rz_remove_config / rz_partition_on."""
+    from . import RedzoneError
+    from .dispatch import method_config
+
+    lib = ctx._lib
+    h = ctx.handle
+    remove, remove_cfg, part = lib.rz_remove, lib.rz_remove_config, lib.rz_partition_on
+    cfgs = {}
+    for (d, r, n, k, e, _), m in zip(rows, methods):
+        if m == 3:
+            st = part(h, d, n, e, r, k, 0, stream)
+        elif m == 0:
+            st = remove(h, d, n, e, r, k, 0, stream)
+        else:
+            key = (m, e)
+            if key not in cfgs:
+                cfgs[key] = ctypes.byref(method_config(m, e))
+            st = remove_cfg(h, d, n, e, r, k, 0, cfgs[key], stream)
+        if st:
+            raise RedzoneError(st, "per-list launch")
+
+
+def _stream_arg(ctx):
     import torch
 
-    from .dispatch import rule_choice, run_method
+    return ctypes.c_void_p(torch.cuda.current_stream(ctx.device).cuda_stream)
 
+
+def _as_descs(ctx, lists):
+    from . import ListDescs
+
+    return lists if isinstance(lists, ListDescs) else ctx.descs(lists)
+
+
+def run_plain(ctx, lists, specs=None):
+    """Per-list launches, method by the k < n/2 rule; wall time (s) from first
+    launch to the end.  lists: [(data, removals), ...] or a ListDescs (built
+    before the clock starts, as a native caller holds its descriptors)."""
+    import torch
+
+    from .dispatch import rule_choice
+
+    descs = _as_descs(ctx, lists)
+    rows = descs.rows.tolist()
+    methods = [rule_choice(n, k) for (_, _, n, k, _, _) in rows]
+    s = _stream_arg(ctx)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for (A, R), (n, k, _, _) in zip(lists, specs):
-        run_method(ctx, rule_choice(n, k), A, R)
+    _launch_each(ctx, rows, methods, s)
     torch.cuda.synchronize()
     return time.perf_counter() - t0
 
@@ -158,24 +200,26 @@ def run_plain(ctx, lists, specs):
 BATCH_MAX_K = 1 << 20
 
 
-def run_batched_learned(ctx, disp, lists, specs, batch_max_k=BATCH_MAX_K):
+def run_batched_learned(ctx, disp, lists, specs=None, batch_max_k=BATCH_MAX_K):
     """Features + MLP choice on the device; lists below batch_max_k removals in
     one persistent launch, the others with their chosen method; wall time (s)
-    of everything including the decision and the batch plan."""
+    of everything including the decision, the descriptor upload and the batch
+    plan.  lists: [(data, removals), ...] or a ListDescs."""
     import torch
 
-    from .dispatch import run_method
-
+    descs = _as_descs(ctx, lists)
+    k = descs.rows["k"]
+    small = np.nonzero(k < batch_max_k)[0]
+    large = np.nonzero(k >= batch_max_k)[0]
+    s = _stream_arg(ctx)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    X = disp.features(lists)
-    choice = disp.choose(X)
-    small = [i for i in range(len(lists)) if specs[i][1] < batch_max_k]
-    if small:
-        ctx.remove_batched([lists[i] for i in small])
-    for i in range(len(lists)):
-        if specs[i][1] >= batch_max_k:
-            run_method(ctx, int(choice[i]), lists[i][0], lists[i][1])
+    X = disp.features(descs)
+    choice = disp.choose(X, descs.rows["n"], k)
+    if len(small):
+        ctx.remove_batched(descs.subset(small))
+    if len(large):
+        _launch_each(ctx, descs.rows[large].tolist(), [int(choice[i]) for i in large], s)
     torch.cuda.synchronize()
     return time.perf_counter() - t0, choice
 
@@ -210,6 +254,7 @@ def evaluate(ctx, disp, specs, seed, reps=3, budget=8 << 30, log=None, sweep=(1 
             tot["redzone_only_us"] += t[0]
             picks[learned[j]] += 1
         work = [(torch.empty_like(A), torch.empty_like(R)) for A, R in pristine]
+        wdescs = ctx.descs(work)
 
         def restore():
             for (a, r), (a0, r0) in zip(work, pristine):
@@ -218,13 +263,13 @@ def evaluate(ctx, disp, specs, seed, reps=3, budget=8 << 30, log=None, sweep=(1 
             torch.cuda.synchronize()
 
         restore()
-        tot["plain_rule_wall_s"] += run_plain(ctx, work, sub)
+        tot["plain_rule_wall_s"] += run_plain(ctx, wdescs)
         restore()
-        wall, _ = run_batched_learned(ctx, disp, work, sub)
+        wall, _ = run_batched_learned(ctx, disp, wdescs)
         tot["batched_learned_wall_s"] += wall
         for b in swept:
             restore()
-            swept[b] += run_batched_learned(ctx, disp, work, sub, batch_max_k=b)[0]
+            swept[b] += run_batched_learned(ctx, disp, wdescs, batch_max_k=b)[0]
         del pristine, work
         torch.cuda.empty_cache()
         if log:
