@@ -35,12 +35,16 @@ def test_batched_learned_path_matches_oracle(ctx, oracle):
     lists = [MX.build_list(ctx, s, 900 + 7 * i) for i, s in enumerate(specs)]
     host = [(A.cpu().numpy(), R.cpu().numpy()) for A, R in lists]
     announce("run batched learned")
-    _, choice = MX.run_batched_learned(ctx, disp, lists, specs)
+    # a threshold inside the stream's k range, so both the batch and the per-list launches run
+    ks = sorted(s[1] for s in specs)
+    bmk = ks[len(ks) // 2]
+    _, choice = MX.run_batched_learned(ctx, disp, lists, specs, batch_max_k=bmk)
     torch.cuda.synchronize()
     for i, ((A0, R0), (A, _)) in enumerate(zip(host, lists)):
         n, k = specs[i][0], specs[i][1]
         got = A.cpu().numpy()
-        if choice[i] == 3:
+        # lists below the batch threshold run in the (Redzone) batch whatever the choice
+        if choice[i] == 3 and k >= bmk:
             bad, fb = oracle.check(A0, got, R0, mode=1)
             assert bad == 0, (i, specs[i], bad, fb)
         else:
