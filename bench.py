@@ -9,8 +9,9 @@ Workload (one step = one removal call on one list resident in HBM):
   scrub are outside the timed call.
 
 Printed: ONE JSON line on stdout.  `value` is list items per second of the
-removal call (n / t).  `roofline` is for the dominant kernel (rz_move), timed
-live with events on the launch stream; `cpu_baseline` is the oracle's
+removal call (n / t).  `roofline` is for the dominant kernel of the launch
+form the tuned table picks (rz_move_pair on this shape), timed live with the
+library's events on the launch stream; `cpu_baseline` is the oracle's
 restatement of Redzone_CPU timed on this host's cores (n/t, same workload).
 Everything else goes to stderr.
 """
@@ -45,7 +46,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-extra", action="store_true", help="skip the comparator measurements")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
-    p.add_argument("--variant", type=int, default=None, help="0 stream kernels, 1 cooperative, 2 persistent queue")
+    p.add_argument("--variant", type=int, default=None,
+                   help="0 three kernels, 1 cooperative, 2 persistent queue, 3 ordered, 4 fused, 5 dense")
     p.add_argument("--wg", type=int, default=256)
     p.add_argument("--items", type=int, default=None)
     p.add_argument("--max-slices", type=int, default=2048)
@@ -65,7 +67,7 @@ def tile_config(args, rz):
 
 def algorithmic_bytes(R, n, k, s, torch):
     """B = 4k (read R) + s*k (read the tail) + s*(k - k_z) (write the moved items),
-    and the move kernel's share (orphan pairs are moved by the pairing kernel)."""
+    and the move kernel's share when a separate kernel pairs the orphans."""
     sz = n - k
     tail = R >= sz
     kz = int(tail.sum().item())
@@ -75,6 +77,15 @@ def algorithmic_bytes(R, n, k, s, torch):
     B = 4 * k + s * k + s * (k - kz)
     B_move = (4 + s) * k + s * (k - kz - pairs)
     return B, B_move, kz, pairs
+
+
+# dominant kernel of each launch form on the headline shape, and whether it
+# also moves the orphan pairs (then its algorithmic bytes are all of B)
+KERNEL_OF_VARIANT = {0: ("rz_move", False), 4: ("rz_move_pair", True), 5: ("rz_dense_scatter", True)}
+PATH_OF_VARIANT = {0: "rz_remove (mark, move, pair kernels)", 1: "rz_remove (one cooperative kernel)",
+                   2: "rz_remove (persistent queue kernel)", 3: "rz_remove (one ordered launch)",
+                   4: "rz_remove (mark kernel, then move with in-kernel orphan pairing)",
+                   5: "rz_remove (dense form: bucketed marks and region-grouped writes)"}
 
 
 class Scrubber:
@@ -240,6 +251,9 @@ def main():
     n = 1 << args.log2n
     k = int(n * args.frac)
     s = args.elem_bytes
+    used = cfg if cfg is not None else ctx.table(s, k.bit_length() - 1)
+    kernel, kernel_pairs = KERNEL_OF_VARIANT.get(used.variant, ("rz_move", False))
+    log(f"launch shape {used.as_tuple()} -> dominant kernel {kernel}")
     log(f"device {info}, n=2^{args.log2n}, k={k}, elem {s} B, pattern {args.pattern}")
     A0 = W.payload(s, n, seed=2026, device="cuda")
     R0 = W.removals(args.pattern, n, k, seed=2027, device="cuda")
@@ -280,10 +294,11 @@ def main():
     bit_exact = bool(np.array_equal(got[: n - k], want[: n - k]))
     log(f"bit-exact against the oracle: {bit_exact}")
 
-    achieved = B_move / (move_ms * 1e-3) / 1e9
+    B_kernel = B if kernel_pairs else B_move
+    achieved = B_kernel / (move_ms * 1e-3) / 1e9
     op_gbs = B / (t_med * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_rz_move_u32_n2e28_k1pct.json")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{kernel}_u32_n2e28_k1pct.json")
     if os.path.exists(tpath) and args.log2n == 28 and args.frac == 0.01 and s == 4 and args.pattern == "uniform":
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -303,10 +318,10 @@ def main():
         "config": {"workload": f"redzone remove, n=2^{args.log2n} {s}-byte items, k/n={args.frac:g} {args.pattern}",
                    "n": n, "k": k, "k_z": kz, "orphan_pairs": pairs, "elem_bytes": s,
                    "cache": "warm" if args.warm else "cold (1 GiB read scrub before each call)",
-                   "path": "rz_remove (mark, move, pair kernels)"},
-        "roofline": {"bound": "hbm", "kernel": "rz_move", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                   "path": PATH_OF_VARIANT.get(used.variant, "rz_remove"), "launch_shape": used.as_tuple()},
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes": B_move, "avg_ms": round(move_ms, 5)},
+                     "algorithmic_bytes": B_kernel, "avg_ms": round(move_ms, 5)},
         "op": {"algorithmic_bytes": B, "GBps": round(op_gbs, 1), "hbm_frac": round(op_gbs / HBM_PEAK_GBS, 4),
                "removals_per_s_G": round(k / (t_med * 1e-3) / 1e9, 3), "bit_exact": bit_exact,
                "kernel_us": {"mark": round(mark_ms * 1e3, 2), "move": round(move_ms * 1e3, 2),
