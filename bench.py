@@ -54,7 +54,29 @@ def parse():
     p.add_argument("--slice-steps", type=int, default=2)
     p.add_argument("--no-batch", action="store_true", help="skip the batched (4096-list) comparison")
     p.add_argument("--mixed-lists", type=int, default=10000, help="lists of the mixed-stream evaluation (0 = skip)")
+    p.add_argument("--chunk", type=int, default=1, help="with --variant: the config's chunk field (dense form: 2 = "
+                   "bucketed tail marks)")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run the counter calibration: copy, add to and sum a 1 GiB int32 tensor, three times")
     return p.parse_args()
+
+
+def copy_calibration(torch):
+    """Kernels of known traffic for scaling FETCH_SIZE / WRITE_SIZE: a 1 GiB blit
+    copy (read 1 GiB, write 1 GiB), an in-place add (the same) and a sum."""
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    evict = torch.ones(n, dtype=torch.int32, device="cuda")
+    for _ in range(3):
+        evict.add_(1)
+        b.copy_(a)
+        evict.add_(1)
+        b.add_(1)
+        evict.add_(1)
+        t = a.sum()
+        torch.cuda.synchronize()
+    log(f"calibration done ({int(t.item())})")
 
 
 def tile_config(args, rz):
@@ -62,7 +84,7 @@ def tile_config(args, rz):
     if args.variant is None:
         return None
     items = args.items or (8 if args.variant == 2 else 4)
-    return rz.TileConfig(args.wg, items, args.max_slices, args.variant, args.slice_steps, 1)
+    return rz.TileConfig(args.wg, items, args.max_slices, args.variant, args.slice_steps, args.chunk)
 
 
 def algorithmic_bytes(R, n, k, s, torch):
@@ -80,8 +102,10 @@ def algorithmic_bytes(R, n, k, s, torch):
 
 
 # dominant kernel of each launch form on the headline shape, and whether it
-# also moves the orphan pairs (then its algorithmic bytes are all of B)
-KERNEL_OF_VARIANT = {0: ("rz_move", False), 4: ("rz_move_pair", True), 5: ("rz_dense_scatter", True)}
+# also moves the orphan pairs (then its algorithmic bytes are all of B); for
+# the dense form the timed span is all of its kernels after the mark
+# ("rz_dense_all")
+KERNEL_OF_VARIANT = {0: ("rz_move", False), 4: ("rz_move_pair", True), 5: ("rz_dense_all", True)}
 PATH_OF_VARIANT = {0: "rz_remove (mark, move, pair kernels)", 1: "rz_remove (one cooperative kernel)",
                    2: "rz_remove (persistent queue kernel)", 3: "rz_remove (one ordered launch)",
                    4: "rz_remove (mark kernel, then move with in-kernel orphan pairing)",
@@ -242,6 +266,9 @@ def main():
         sys.exit(2)
     import torch
 
+    if args.copy_calibration:
+        copy_calibration(torch)
+        return
     import redzone_amd as rz
     from redzone_amd import workloads as W
 
