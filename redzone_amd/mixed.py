@@ -197,7 +197,7 @@ def run_plain(ctx, lists, specs=None):
 # launch costs about 10 us, a list inside a batch about half a microsecond, and
 # the queue moves large lists at the same random-write rate as their own
 # launch.  Larger lists are launched on their own with the chosen method.
-BATCH_MAX_K = 1 << 22
+BATCH_MAX_K = 1 << 24
 
 
 def run_batched_learned(ctx, disp, lists, specs=None, batch_max_k=BATCH_MAX_K):
@@ -224,7 +224,7 @@ def run_batched_learned(ctx, disp, lists, specs=None, batch_max_k=BATCH_MAX_K):
     return time.perf_counter() - t0, choice
 
 
-def evaluate(ctx, disp, specs, seed, reps=3, budget=8 << 30, log=None, sweep=(1 << 20, 1 << 24)):
+def evaluate(ctx, disp, specs, seed, reps=3, budget=8 << 30, log=None, sweep=(1 << 20, 1 << 22)):
     """Mixed-stream evaluation; returns a dict of totals and regrets.  `sweep`
     lists further batch thresholds whose batched wall time is also measured."""
     import torch
