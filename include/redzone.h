@@ -73,6 +73,23 @@ typedef struct rz_tile_config {
   int32_t chunk;       /* variant 2 and batches: queue items taken per ticket (0 = 1) */
 } rz_tile_config;
 
+/* Removal with the tail marks already set by the caller's producer of the
+ * list (see redzone_device.h: rz_mark_removal).  d_tail_bits holds
+ * ceil(k/32) words with bit r-(n-k) set for every removal r >= n-k and no
+ * other bit; the call skips the mark step and leaves the bitmap zero.  cfg may
+ * be null (the tuned table); shapes whose single launch includes the mark
+ * (variants 1-3) run as variant 4.  RZ_SORTED is ignored and RZ_STATS counts
+ * no tail removals (the marks were not made here).  Has no counterpart in the
+ * reference: its phase 1 always scans the list (redzone.hxx:698-707). */
+rz_status rz_remove_premarked(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, int32_t* d_removals,
+                              int64_t k, uint32_t* d_tail_bits, uint32_t flags, const rz_tile_config* cfg,
+                              void* stream);
+/* The mark step alone, into a caller-owned tail bitmap (zero beforehand),
+ * e.g. on another stream while the list is still being written; built on
+ * rz_mark_removal.  redzone.hxx:698-707 without the data sentinel. */
+rz_status rz_mark_removals(rz_ctx* ctx, const int32_t* d_removals, int64_t n, int64_t k, uint32_t* d_tail_bits,
+                           void* stream);
+
 /* Statistics of the last call made with RZ_STATS. */
 typedef struct rz_stats {
   int64_t n, k;
@@ -239,6 +256,27 @@ rz_status rz_mlp_grad(rz_ctx* ctx, const float* d_params, const float* d_x, cons
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with step >= 1. */
 rz_status rz_mlp_adam(rz_ctx* ctx, float* d_params, const float* d_grad, float* d_m, float* d_v, int32_t step,
                       float lr, void* stream);
+
+/* Methods of the dispatcher (outputs of the network, in order). */
+#define RZ_METHOD_REDZONE 0   /* rz_remove with the tuned launch shape */
+#define RZ_METHOD_ORDERED 1   /* one ordered launch (variant 3) */
+#define RZ_METHOD_DENSE 2     /* the dense form (variant 5) */
+#define RZ_METHOD_PARTITION 3 /* rz_partition_on */
+/* Launch shape of RZ_METHOD_ORDERED / RZ_METHOD_DENSE for an element size
+ * (RZ_EINVAL for the other methods, which have none).  Needs no context. */
+rz_status rz_dispatch_method_config(int32_t method, uint32_t elem_bytes, rz_tile_config* out);
+/* Installs the network's parameters (host array of RZ_MLP_PARAMS floats) in
+ * the context; null removes them, and rz_remove_auto then uses the k < n/2
+ * rule.  Synchronous. */
+rz_status rz_dispatch_set_params(rz_ctx* ctx, const float* h_params);
+/* The reference's remove(data, n, removeList, k) with the method chosen per
+ * list: the features of the list and the network's argmin are computed on
+ * the stream, the choice is read back (this synchronises the stream), then
+ * the chosen method runs on the stream.  *method (may be null) gets the
+ * RZ_METHOD_* used.  Replaces the fixed choice the reference states
+ * (README.md:6; paper p.14: Redzone below k = n/2, O(n) above). */
+rz_status rz_remove_auto(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, int32_t* d_removals, int64_t k,
+                         uint32_t flags, int32_t* method, void* stream);
 
 #ifdef __cplusplus
 }

@@ -14,11 +14,13 @@ import ctypes
 import os
 
 from . import _native
-from ._native import (RZ_COOPERATIVE, RZ_PRESERVE_REMOVALS, RZ_PROFILE, RZ_SORTED, RZ_STATS, RZ_VALIDATE, BatchInfo, ListDesc,
+from ._native import (RZ_COOPERATIVE, RZ_METHOD_DENSE, RZ_METHOD_ORDERED, RZ_METHOD_PARTITION, RZ_METHOD_REDZONE,
+                      RZ_PRESERVE_REMOVALS, RZ_PROFILE, RZ_SORTED, RZ_STATS, RZ_VALIDATE, BatchInfo, ListDesc,
                       RedzoneError, Stats, TileConfig)
 
 __all__ = ["Context", "Batch", "ListDescs", "remove", "partition", "TileConfig", "Stats", "RedzoneError", "RZ_PRESERVE_REMOVALS",
-           "RZ_VALIDATE", "RZ_COOPERATIVE", "RZ_STATS", "RZ_PROFILE", "RZ_SORTED", "elem_bytes_of", "default_context"]
+           "RZ_VALIDATE", "RZ_COOPERATIVE", "RZ_STATS", "RZ_PROFILE", "RZ_SORTED", "RZ_METHOD_REDZONE",
+           "RZ_METHOD_ORDERED", "RZ_METHOD_DENSE", "RZ_METHOD_PARTITION", "elem_bytes_of", "default_context"]
 
 
 def elem_bytes_of(data):
@@ -49,7 +51,7 @@ def _check_tensor(t, name, device):
 class Context:
     """Owns an rz_ctx (device scratch + tuning table) on one device."""
 
-    def __init__(self, device=0, load_tuned=True):
+    def __init__(self, device=0, load_tuned=True, load_dispatcher=True):
         self._lib = _native.load()
         self.device = int(device)
         h = ctypes.c_void_p()
@@ -58,9 +60,18 @@ class Context:
             raise RedzoneError(st, "rz_ctx_create")
         self._h = h
         self.tuned_table = None
-        if load_tuned and os.path.exists(_native.TUNED_TABLE) and self.device_info()["arch"].startswith("gfx950"):
+        self.dispatcher_model = None
+        gfx950 = self.device_info()["arch"].startswith("gfx950")
+        if load_tuned and os.path.exists(_native.TUNED_TABLE) and gfx950:
             self.load_table(_native.TUNED_TABLE)
             self.tuned_table = _native.TUNED_TABLE
+        model = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dispatch_model_gfx950.npz")
+        if load_dispatcher and os.path.exists(model) and gfx950:
+            import numpy as np
+
+            with np.load(model, allow_pickle=False) as z:
+                self.set_dispatch_params(z["params"])
+            self.dispatcher_model = model
 
     def close(self):
         if getattr(self, "_h", None):
@@ -116,6 +127,53 @@ class Context:
             self._call("rz_remove_config", self._h, ctypes.c_void_p(data.data_ptr()), n, esz,
                        ctypes.c_void_p(removals.data_ptr()), k, flags, ctypes.byref(config), s)
         return n - k
+
+    def mark_removals(self, removals, n, tail_bits, k=None, stream=None):
+        """The mark step alone into tail_bits (int32 device tensor of ceil(k/32)
+        words, zero beforehand): bit r-(n-k) for every removal r >= n-k."""
+        _check_tensor(removals, "removals", self.device)
+        _check_tensor(tail_bits, "tail_bits", self.device)
+        k = removals.shape[0] if k is None else int(k)
+        if tail_bits.numel() * tail_bits.element_size() < 4 * ((k + 31) // 32):
+            raise ValueError("tail_bits holds fewer than ceil(k/32) words")
+        self._call("rz_mark_removals", self._h, ctypes.c_void_p(removals.data_ptr()), int(n), k,
+                   ctypes.c_void_p(tail_bits.data_ptr()), _stream_handle(stream, self.device))
+
+    def remove_premarked(self, data, removals, tail_bits, n=None, k=None, flags=0, config=None, stream=None):
+        """remove() with the tail marks already in tail_bits (see mark_removals and
+        include/redzone_device.h); leaves tail_bits zero.  Returns n - k."""
+        n, k, esz = self._args(data, removals, n, k)
+        _check_tensor(tail_bits, "tail_bits", self.device)
+        if tail_bits.numel() * tail_bits.element_size() < 4 * ((k + 31) // 32):
+            raise ValueError("tail_bits holds fewer than ceil(k/32) words")
+        cfg = ctypes.byref(config) if config is not None else None
+        self._call("rz_remove_premarked", self._h, ctypes.c_void_p(data.data_ptr()), n, esz,
+                   ctypes.c_void_p(removals.data_ptr()), k, ctypes.c_void_p(tail_bits.data_ptr()), flags, cfg,
+                   _stream_handle(stream, self.device))
+        return n - k
+
+    def set_dispatch_params(self, params):
+        """Installs the dispatcher network's parameters (RZ_MLP_PARAMS floats) for
+        remove_auto; None removes them (remove_auto then uses the k < n/2 rule)."""
+        if params is None:
+            self._call("rz_dispatch_set_params", self._h, None)
+            return
+        import numpy as np
+
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != 2660:
+            raise ValueError("the dispatcher has 2660 parameters")
+        self._call("rz_dispatch_set_params", self._h, p.ctypes.data_as(ctypes.c_void_p))
+
+    def remove_auto(self, data, removals, n=None, k=None, flags=0, stream=None):
+        """remove() with the method picked per list by the installed network (or
+        the k < n/2 rule); returns (n - k, RZ_METHOD_* used).  Reads the choice
+        back, so it synchronises the stream."""
+        n, k, esz = self._args(data, removals, n, k)
+        m = ctypes.c_int32(-1)
+        self._call("rz_remove_auto", self._h, ctypes.c_void_p(data.data_ptr()), n, esz,
+                   ctypes.c_void_p(removals.data_ptr()), k, flags, ctypes.byref(m), _stream_handle(stream, self.device))
+        return n - k, m.value
 
     def partition(self, data, removals, n=None, k=None, flags=0, stream=None):
         """O(n) stable in-place partition; survivors keep their order.  Returns n - k."""

@@ -18,6 +18,8 @@ RZ_COOPERATIVE = 0x4
 RZ_STATS = 0x8
 RZ_PROFILE = 0x10
 RZ_SORTED = 0x20
+# dispatcher methods (RZ_METHOD_*)
+RZ_METHOD_REDZONE, RZ_METHOD_ORDERED, RZ_METHOD_DENSE, RZ_METHOD_PARTITION = range(4)
 
 
 class TileConfig(ctypes.Structure):
@@ -119,6 +121,11 @@ SIGNATURES = [
     ("rz_mlp_forward", _i32, [_vp, _vp, _vp, _i32, _vp, _vp, _vp]),
     ("rz_mlp_grad", _i32, [_vp, _vp, _vp, _vp, _vp, _i32, ctypes.c_float, _vp, _vp, _vp, _vp]),
     ("rz_mlp_adam", _i32, [_vp, _vp, _vp, _vp, _vp, _i32, ctypes.c_float, _vp]),
+    ("rz_remove_premarked", _i32, [_vp, _vp, _i64, _u32, _vp, _i64, _vp, _u32, ctypes.POINTER(TileConfig), _vp]),
+    ("rz_mark_removals", _i32, [_vp, _vp, _i64, _i64, _vp, _vp]),
+    ("rz_dispatch_method_config", _i32, [_i32, _u32, ctypes.POINTER(TileConfig)]),
+    ("rz_dispatch_set_params", _i32, [_vp, _vp]),
+    ("rz_remove_auto", _i32, [_vp, _vp, _i64, _u32, _vp, _i64, _u32, ctypes.POINTER(_i32), _vp]),
 ]
 
 TUNED_TABLE = os.path.join(_HERE, "tuned_gfx950.txt")

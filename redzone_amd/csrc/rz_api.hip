@@ -128,10 +128,11 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
   DeviceBuffer* bufs[] = {&ctx->tail_bits, &ctx->list_bits, &ctx->counts, &ctx->orphans,
                           &ctx->status,    &ctx->ctrl,      &ctx->queue,  &ctx->mstatus, &ctx->slots,
                           &ctx->dense_cnt, &ctx->dense_small, &ctx->dense_ent, &ctx->dense_fine,
-                          &ctx->dense_src};
+                          &ctx->dense_src, &ctx->dispatch};
   for (DeviceBuffer* b : bufs)
     if (b->ptr) (void)hipFree(b->ptr);
   if (ctx->ctrl_host) (void)hipHostFree(ctx->ctrl_host);
+  if (ctx->auto_host) (void)hipHostFree(ctx->auto_host);
   for (hipEvent_t e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   delete ctx;
@@ -189,6 +190,38 @@ rz_status rz_remove(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, i
   if (k == 0 || k == n) return RZ_OK;
   const rz_tile_config cfg = ctx->table[esz_index(elem_bytes)][log2_floor(k)];
   return rz_remove_config(ctx, d_data, n, elem_bytes, d_removals, k, flags, &cfg, stream);
+}
+
+rz_status rz_remove_premarked(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, int32_t* d_removals,
+                              int64_t k, uint32_t* d_tail_bits, uint32_t flags, const rz_tile_config* cfg,
+                              void* stream) {
+  rz_status st = check_args(ctx, d_data, n, elem_bytes, d_removals, k);
+  if (st != RZ_OK) return st;
+  if (k > 0 && (!d_tail_bits || ((uintptr_t)d_tail_bits % 4) != 0)) return RZ_EINVAL;
+  if (k == 0) return RZ_OK;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  DeviceGuard g(ctx->device);
+  hipStream_t s = (hipStream_t)stream;
+  if (k == n) return fill_u32(d_tail_bits, 0u, (size_t)((k + 31) / 32), s);  // nothing moves; leave the bitmap zero
+  if (flags & RZ_VALIDATE) {
+    if ((st = launch_validate(ctx, d_removals, n, k, flags, s)) != RZ_OK) return st;
+  }
+  const rz_tile_config c = cfg ? *cfg : ctx->table[esz_index(elem_bytes)][log2_floor(k)];
+  st = launch_plain(ctx, d_data, n, elem_bytes, d_removals, k, (flags & ~(uint32_t)RZ_SORTED) | kFlagPremarked, c, s,
+                    d_tail_bits);
+  if (st == RZ_OK && (flags & RZ_STATS)) {
+    RZ_HIP(hipMemcpyAsync(ctx->ctrl_host, ctx->ctrl.ptr, kCtrlWords * 4, hipMemcpyDeviceToHost, s));
+    RZ_HIP(hipStreamSynchronize(s));
+  }
+  return st;
+}
+
+rz_status rz_mark_removals(rz_ctx* ctx, const int32_t* d_removals, int64_t n, int64_t k, uint32_t* d_tail_bits,
+                           void* stream) {
+  if (!ctx || n < 0 || k < 0 || k > n || n >= (1ll << 31)) return RZ_EINVAL;
+  if (k > 0 && (!d_removals || !d_tail_bits)) return RZ_EINVAL;
+  DeviceGuard g(ctx->device);
+  return launch_mark_removals(ctx, d_removals, n, k, d_tail_bits, (hipStream_t)stream);
 }
 
 rz_status rz_remove_u32(rz_ctx* ctx, uint32_t* d_data, int64_t n, int32_t* d_removals, int64_t k, uint32_t flags,

@@ -19,7 +19,8 @@ enum CtrlWord : int {
   kCtrlStatTail = 128,    // uint64: tail removals k_z (RZ_STATS)
   kCtrlStatOrphans = 160, // uint64: orphan pairs (RZ_STATS)
   kCtrlValidate = 192,    // validation flags: 1 = out of range, 2 = duplicate, 4 = not ascending
-  kCtrlMoveEpoch = 224,   // call counter of the fused move-and-pair form (bumped by its mark kernel)
+  kCtrlMoveEpoch = 224,   // call counter of the fused move-and-pair form (bumped by its last workgroup)
+  kCtrlMoveExit = 240,    // workgroups of the fused kernel that have finished (reset by the last)
   kCtrlWords = 256
 };
 
@@ -28,6 +29,9 @@ enum ErrorBits : uint32_t {
   kErrBarrierTimeout = 2u,  // an in-kernel wait gave up (workgroups not co-resident)
   kErrLookbackTimeout = 4u, // a look-back wait gave up
 };
+
+// internal flag: the tail bitmap was set by the caller (rz_remove_premarked)
+constexpr uint32_t kFlagPremarked = 0x80000000u;
 
 constexpr int kMaxSlices = 4096;  // cap on move-kernel workgroups (LDS prefix in the pairing kernel)
 
@@ -60,6 +64,9 @@ struct rz_ctx {
   rz::DeviceBuffer dense_ent;    // dense form: coarse-bin head writes (positions, then values)
   rz::DeviceBuffer dense_fine;   // dense form: fine-bin head writes (fixed capacity per bin)
   rz::DeviceBuffer dense_src;    // dense form: source orphan values by rank
+  rz::DeviceBuffer dispatch;   // network parameters, one list descriptor, its features and the choice
+  bool dispatch_ready = false; // parameters installed (else the k < n/2 rule)
+  uint8_t* auto_host = nullptr;  // pinned staging of the descriptor and the choice
   rz_tile_config batch_cfg;    // default launch shape of batches
   rz_batch* last_batch = nullptr;  // plan of the last rz_remove_batched call
   uint32_t* ctrl_host = nullptr;  // pinned mirror for stats / validation reads
@@ -81,8 +88,10 @@ inline int log2_floor(int64_t v) {
 rz_tile_config default_config(uint32_t esz, int64_t n, int64_t k);
 
 // launchers (one per translation unit)
+// ext_bits: a caller-owned tail bitmap (with kFlagPremarked: already marked)
 rz_status launch_plain(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, int32_t* R, int64_t k, uint32_t flags,
-                       const rz_tile_config& cfg, hipStream_t s);
+                       const rz_tile_config& cfg, hipStream_t s, uint32_t* ext_bits = nullptr);
+rz_status launch_mark_removals(rz_ctx* ctx, const int32_t* R, int64_t n, int64_t k, uint32_t* bits, hipStream_t s);
 rz_status launch_partition(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, const int32_t* R, int64_t k,
                            uint32_t flags, hipStream_t s);
 rz_status launch_validate(rz_ctx* ctx, const int32_t* R, int64_t n, int64_t k, uint32_t flags, hipStream_t s);

@@ -8,6 +8,8 @@
 // for each layer, in order.  Training minimises the masked mean squared error
 // on measured log times with Adam.  Per-sample gradients are written out and
 // summed in a fixed order, so a gradient is the same bits on every run.
+#include <mutex>
+
 #include "rz_ctx.hpp"
 #include "rz_device.hpp"
 
@@ -269,6 +271,96 @@ rz_status rz_mlp_adam(rz_ctx* ctx, float* d_params, const float* d_grad, float* 
                      d_m, d_v, lr, b1, b2, eps, c1, c2);
   RZ_HIP(hipGetLastError());
   return RZ_OK;
+}
+
+rz_status rz_dispatch_method_config(int32_t method, uint32_t elem_bytes, rz_tile_config* out) {
+  if (!out || esz_index(elem_bytes) < 0) return RZ_EINVAL;
+  if (method == RZ_METHOD_ORDERED) {
+    *out = rz_tile_config{256, 4, 4096, 3, 0, 1};
+    return RZ_OK;
+  }
+  if (method == RZ_METHOD_DENSE) {
+    // 16-byte items stage half as many entries per step (LDS)
+    *out = rz_tile_config{1024, elem_bytes == 16 ? 4 : 8, 1024, 5, 11, 2};
+    return RZ_OK;
+  }
+  return RZ_EINVAL;
+}
+
+// dispatch buffer: [params NP floats | pad to 256 B | descriptor | features | choice]
+static constexpr size_t kDispParams = ((size_t)NP * 4 + 255) & ~(size_t)255;
+static constexpr size_t kDispDesc = kDispParams;
+static constexpr size_t kDispFeat = kDispDesc + 256;
+static constexpr size_t kDispChoice = kDispFeat + 256;
+static constexpr size_t kDispBytes = kDispChoice + 256;
+
+rz_status rz_dispatch_set_params(rz_ctx* ctx, const float* h_params) {
+  if (!ctx) return RZ_EINVAL;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  int prev = -1;
+  (void)hipGetDevice(&prev);
+  if (prev != ctx->device) (void)hipSetDevice(ctx->device);
+  rz_status st = RZ_OK;
+  if (!h_params) {
+    ctx->dispatch_ready = false;
+  } else if ((st = ensure(ctx, ctx->dispatch, kDispBytes, nullptr)) == RZ_OK) {
+    if (hipMemcpy(ctx->dispatch.ptr, h_params, (size_t)NP * 4, hipMemcpyHostToDevice) != hipSuccess) st = RZ_EHIP;
+    ctx->dispatch_ready = st == RZ_OK;
+  }
+  if (prev >= 0 && prev != ctx->device) (void)hipSetDevice(prev);
+  return st;
+}
+
+// The network's choice for one list (stream-ordered, then read back).
+static rz_status auto_choose(rz_ctx* ctx, void* d_data, int64_t n, uint32_t esz, int32_t* d_removals, int64_t k,
+                             hipStream_t s, int32_t* method) {
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (!ctx->dispatch_ready) {
+    *method = 2 * k < n ? RZ_METHOD_REDZONE : RZ_METHOD_PARTITION;
+    return RZ_OK;
+  }
+  int prev = -1;
+  (void)hipGetDevice(&prev);
+  if (prev != ctx->device) (void)hipSetDevice(ctx->device);
+  rz_status st = RZ_OK;
+  if (!ctx->auto_host && hipHostMalloc((void**)&ctx->auto_host, 256, hipHostMallocDefault) != hipSuccess) {
+    ctx->auto_host = nullptr;
+    st = RZ_ENOMEM;
+  }
+  if (st == RZ_OK) {
+    char* base = (char*)ctx->dispatch.ptr;
+    rz_list_desc* hd = (rz_list_desc*)ctx->auto_host;
+    *hd = rz_list_desc{d_data, d_removals, n, k, esz, 0u};
+    int32_t* hc = (int32_t*)(ctx->auto_host + 128);
+    *hc = -1;
+    if (hipMemcpyAsync(base + kDispDesc, hd, sizeof(rz_list_desc), hipMemcpyHostToDevice, s) != hipSuccess) st = RZ_EHIP;
+    if (st == RZ_OK)
+      st = rz_features(ctx, (const rz_list_desc*)(base + kDispDesc), 1, (float*)(base + kDispFeat), s);
+    if (st == RZ_OK)
+      st = rz_mlp_forward(ctx, (const float*)base, (const float*)(base + kDispFeat), 1, nullptr,
+                          (int32_t*)(base + kDispChoice), s);
+    if (st == RZ_OK && hipMemcpyAsync(hc, base + kDispChoice, 4, hipMemcpyDeviceToHost, s) != hipSuccess) st = RZ_EHIP;
+    if (st == RZ_OK && hipStreamSynchronize(s) != hipSuccess) st = RZ_EHIP;
+    if (st == RZ_OK) *method = (*hc >= 0 && *hc < RZ_MLP_OUTPUTS) ? *hc : (2 * k < n ? 0 : 3);
+  }
+  if (prev >= 0 && prev != ctx->device) (void)hipSetDevice(prev);
+  return st;
+}
+
+rz_status rz_remove_auto(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, int32_t* d_removals, int64_t k,
+                         uint32_t flags, int32_t* method, void* stream) {
+  if (!ctx || n < 0 || k < 0 || k > n || esz_index(elem_bytes) < 0) return RZ_EINVAL;
+  if (k > 0 && (!d_data || !d_removals)) return RZ_EINVAL;
+  int32_t m = RZ_METHOD_REDZONE;
+  rz_status st = RZ_OK;
+  if (k > 0 && k < n) st = auto_choose(ctx, d_data, n, elem_bytes, d_removals, k, (hipStream_t)stream, &m);
+  if (st != RZ_OK) return st;
+  if (method) *method = m;
+  if (m == RZ_METHOD_PARTITION) return rz_partition_on(ctx, d_data, n, elem_bytes, d_removals, k, flags, stream);
+  if (m == RZ_METHOD_REDZONE) return rz_remove(ctx, d_data, n, elem_bytes, d_removals, k, flags, stream);
+  rz_tile_config cfg;
+  if ((st = rz_dispatch_method_config(m, elem_bytes, &cfg)) != RZ_OK) return st;
+  return rz_remove_config(ctx, d_data, n, elem_bytes, d_removals, k, flags, &cfg, stream);
 }
 
 }  // extern "C"

@@ -33,16 +33,31 @@
 // out-of-bounds access: marks are taken only for indices inside the tail,
 // moves only into holes below n-k, and pairing stops at the smaller orphan
 // total (a mismatch sets an error bit in the control block).
+#include "../../include/redzone_device.h"
 #include "rz_redzone_dev.hpp"
 
 namespace rz {
 
-// `epoch` (fused form only, else null) is bumped once per call.
+// The mark step for callers that mark into their own bitmap (rz_mark_removals),
+// written with the public producer helper.
+__global__ __launch_bounds__(256) void rz_mark_removals_kernel(const int32_t* __restrict__ R, int64_t n, int64_t k,
+                                                                uint32_t* __restrict__ bits) {
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < k; i += (int64_t)gridDim.x * 256ll)
+    rz_mark_removal(bits, R[i], n, k);
+}
+
+rz_status launch_mark_removals(rz_ctx* ctx, const int32_t* R, int64_t n, int64_t k, uint32_t* bits, hipStream_t s) {
+  if (k == 0) return RZ_OK;
+  const int64_t g = (k + 1023) / 1024;
+  const int64_t cap = (int64_t)ctx->cu_count * 8;
+  hipLaunchKernelGGL(rz_mark_removals_kernel, dim3((uint32_t)(g < cap ? g : cap)), dim3(256), 0, s, R, n, k, bits);
+  RZ_HIP(hipGetLastError());
+  return RZ_OK;
+}
+
 template <bool STATS>
 __global__ __launch_bounds__(256) void rz_mark_tail(const int32_t* __restrict__ R, uint32_t k, uint32_t sZ,
-                                                     uint32_t* __restrict__ bits, uint64_t* __restrict__ stat_tail,
-                                                     uint32_t* __restrict__ epoch) {
-  if (epoch && blockIdx.x == 0 && threadIdx.x == 0) add_relaxed(epoch, 1u);
+                                                     uint32_t* __restrict__ bits, uint64_t* __restrict__ stat_tail) {
   mark_body<STATS>(R, k, k, sZ, bits, stat_tail, blockIdx.x * 256u + threadIdx.x, gridDim.x * 256u);
 }
 
@@ -51,8 +66,7 @@ __global__ __launch_bounds__(256) void rz_mark_tail(const int32_t* __restrict__ 
 // chunk's last entry is below the red zone, so only the suffix is read.
 constexpr uint32_t kSortedChunk = 4096;
 __global__ __launch_bounds__(256) void rz_mark_tail_sorted(const int32_t* __restrict__ R, uint32_t k, uint32_t sZ,
-                                                            uint32_t* __restrict__ bits, uint32_t* __restrict__ epoch) {
-  if (epoch && blockIdx.x == 0 && threadIdx.x == 0) add_relaxed(epoch, 1u);
+                                                            uint32_t* __restrict__ bits) {
   const uint32_t lo = blockIdx.x * kSortedChunk;
   const uint32_t hi = min(lo + kSortedChunk, k);
   if ((uint32_t)R[hi - 1] < sZ) return;  // uniform across the workgroup
@@ -96,9 +110,9 @@ __global__ __launch_bounds__(256) void rz_pair(T* __restrict__ A, const int32_t*
 // Status word: [63:62] flag (0 none, 1 aggregate, 2 inclusive prefix),
 // [61:31] dest orphans, [30:0] source orphans (each count is below 2^31, so
 // packed words add without carries between the fields).  The words live in
-// two halves of kMaxSlices used on alternate calls (parity of the epoch that
-// the mark kernel bumps); a call zeroes the half it does not use, so the next
-// call starts from zeros.
+// two halves of kMaxSlices used on alternate calls (parity of the call epoch,
+// which the last workgroup to leave bumps for the next call); a call zeroes
+// the half it does not use, so the next call starts from zeros.
 // ---------------------------------------------------------------------------
 constexpr uint64_t kStAgg = 1ull << 62, kStIncl = 2ull << 62, kStCount = (1ull << 62) - 1;
 
@@ -262,6 +276,12 @@ __global__ __launch_bounds__(WG) void rz_move_pair(T* __restrict__ A, const int3
   }
   // leave the bitmap zero for the next call (slices are word aligned)
   for (uint32_t w = (a >> 5) + tid; w < ((b + 31u) >> 5); w += WG) bits[w] = 0u;
+  // the last workgroup to leave (all of them have read the epoch by then)
+  // starts the next call's epoch
+  if (tid == 0 && add_relaxed(&ctrl[kCtrlMoveExit], 1u) == ns - 1u) {
+    st_relaxed(&ctrl[kCtrlMoveExit], 0u);
+    st_relaxed(&ctrl[kCtrlMoveEpoch], epoch + 1u);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -471,19 +491,19 @@ static rz_status dispatch_coop(int wg, int ipt, rz_ctx* ctx, T* A, int32_t* R, i
 
 template <typename T>
 static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k, uint32_t flags,
-                            const rz_tile_config& cfg, bool coop, hipStream_t s) {
+                            const rz_tile_config& cfg, bool coop, hipStream_t s, uint32_t* ext_bits) {
   const uint32_t sZ = (uint32_t)(n - k);
   uint32_t L, ns;
   slice_shape(k, cfg, kMaxSlices, &L, &ns);
   rz_status st;
-  if ((st = ensure(ctx, ctx->tail_bits, (size_t)((k + 63) / 64) * 8, s)) != RZ_OK) return st;
+  if (!ext_bits && (st = ensure(ctx, ctx->tail_bits, (size_t)((k + 63) / 64) * 8, s)) != RZ_OK) return st;
   if ((st = ensure(ctx, ctx->counts, (size_t)ns * 8, s)) != RZ_OK) return st;
   int32_t* Rd = R;
   if (flags & RZ_PRESERVE_REMOVALS) {
     if ((st = ensure(ctx, ctx->orphans, (size_t)k * 4, s)) != RZ_OK) return st;
     Rd = (int32_t*)ctx->orphans.ptr;
   }
-  uint32_t* bits = (uint32_t*)ctx->tail_bits.ptr;
+  uint32_t* bits = ext_bits ? ext_bits : (uint32_t*)ctx->tail_bits.ptr;
   uint64_t* counts = (uint64_t*)ctx->counts.ptr;
   uint32_t* ctrl = (uint32_t*)ctx->ctrl.ptr;
   const int stats = (flags & RZ_STATS) ? 1 : 0;
@@ -505,26 +525,24 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
     return prof_mark(ctx, flags, 3, s);
   }
   const bool fused = cfg.variant == 4 && fused_fits(ctx, cfg, L);
-  uint32_t* epoch = nullptr;
   if (fused) {
     if ((st = ensure(ctx, ctx->mstatus, 2 * (size_t)kMaxSlices * 8, s)) != RZ_OK) return st;
     if ((st = ensure(ctx, ctx->slots, ((size_t)k / 2 + 1) * 8, s)) != RZ_OK) return st;
-    epoch = ctrl + kCtrlMoveEpoch;
   }
-  const bool dense_marks = dense_bucket_marks(cfg, k, flags);
+  const bool premarked = (flags & kFlagPremarked) != 0;
+  const bool dense_marks = !premarked && dense_bucket_marks(cfg, k, flags);
   const int64_t g1w = (k + 256 * 4 - 1) / (256 * 4);
   const dim3 g1((uint32_t)(g1w < (int64_t)ctx->cu_count * 8 ? g1w : ctx->cu_count * 8));
-  if (dense_marks) {
-    // the dense form sets the tail marks itself
+  if (premarked || dense_marks) {
+    // the caller's producer set the tail marks, or the dense form sets them itself
   } else if (stats)
     hipLaunchKernelGGL(rz_mark_tail<true>, g1, dim3(256), 0, s, R, (uint32_t)k, sZ, bits,
-                       (uint64_t*)(ctrl + kCtrlStatTail), epoch);
+                       (uint64_t*)(ctrl + kCtrlStatTail));
   else if (flags & RZ_SORTED)
     hipLaunchKernelGGL(rz_mark_tail_sorted, dim3((uint32_t)((k + kSortedChunk - 1) / kSortedChunk)), dim3(256), 0,
-                       s, R, (uint32_t)k, sZ, bits, epoch);
+                       s, R, (uint32_t)k, sZ, bits);
   else
-    hipLaunchKernelGGL(rz_mark_tail<false>, g1, dim3(256), 0, s, R, (uint32_t)k, sZ, bits, (uint64_t*)nullptr,
-                       epoch);
+    hipLaunchKernelGGL(rz_mark_tail<false>, g1, dim3(256), 0, s, R, (uint32_t)k, sZ, bits, (uint64_t*)nullptr);
   RZ_HIP(hipGetLastError());
   if ((st = prof_mark(ctx, flags, 1, s)) != RZ_OK) return st;
   if (cfg.variant == 5) {
@@ -552,13 +570,18 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
 }
 
 rz_status launch_plain(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, int32_t* R, int64_t k, uint32_t flags,
-                       const rz_tile_config& cfg, hipStream_t s) {
+                       const rz_tile_config& cfg_in, hipStream_t s, uint32_t* ext_bits) {
+  rz_tile_config cfg = cfg_in;
+  const bool premarked = (flags & kFlagPremarked) != 0;
+  // the single-launch forms include their own mark step: with the marks set
+  // by the caller, run their shape as the two-kernel fused form
+  if (premarked && cfg.variant >= 1 && cfg.variant <= 3) cfg.variant = 4;
   if (cfg.variant == 2) return launch_queue_single(ctx, data, n, esz, R, k, flags, cfg, s);
-  const bool coop = cfg.variant == 1 || (flags & RZ_COOPERATIVE);
+  const bool coop = !premarked && (cfg.variant == 1 || (flags & RZ_COOPERATIVE));
   switch (esz) {
-    case 4: return plain_impl<uint32_t>(ctx, (uint32_t*)data, n, R, k, flags, cfg, coop, s);
-    case 8: return plain_impl<uint64_t>(ctx, (uint64_t*)data, n, R, k, flags, cfg, coop, s);
-    case 16: return plain_impl<uint4>(ctx, (uint4*)data, n, R, k, flags, cfg, coop, s);
+    case 4: return plain_impl<uint32_t>(ctx, (uint32_t*)data, n, R, k, flags, cfg, coop, s, ext_bits);
+    case 8: return plain_impl<uint64_t>(ctx, (uint64_t*)data, n, R, k, flags, cfg, coop, s, ext_bits);
+    case 16: return plain_impl<uint4>(ctx, (uint4*)data, n, R, k, flags, cfg, coop, s, ext_bits);
     default: return RZ_EINVAL;
   }
 }

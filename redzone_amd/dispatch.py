@@ -37,13 +37,15 @@ RULE_REDZONE, RULE_PARTITION = 0, 3
 
 
 def method_config(method, elem_bytes):
-    """Launch shape of a Redzone method (None: the tuned table)."""
-    if method == 1:
-        return TileConfig(256, 4, 4096, 3, 0, 1)
-    if method == 2:
-        # 16-byte items stage half as many entries per step (LDS)
-        return TileConfig(1024, 4 if elem_bytes == 16 else 8, 1024, 5, 11, 2)
-    return None
+    """Launch shape of a Redzone method (None: the tuned table), as the library
+    defines it (rz_dispatch_method_config)."""
+    if method not in (1, 2):
+        return None
+    out = TileConfig()
+    st = _native.load().rz_dispatch_method_config(method, elem_bytes, ctypes.byref(out))
+    if st:
+        raise ValueError(f"no launch shape for method {method}, {elem_bytes}-byte items")
+    return out
 
 MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dispatch_model_gfx950.npz")
 

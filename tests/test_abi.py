@@ -67,3 +67,24 @@ def test_library_is_built_for_gfx950():
 
     blob = open(_native.LIB_PATH, "rb").read()
     assert b"gfx950" in blob
+
+
+def test_dispatch_method_configs():
+    """The dispatcher's fixed launch shapes come from the library (no device needed)."""
+    import ctypes
+
+    from redzone_amd import _native
+    from redzone_amd.dispatch import method_config
+
+    lib = _native.load()
+    out = _native.TileConfig()
+    assert lib.rz_dispatch_method_config(1, 4, ctypes.byref(out)) == 0
+    assert out.as_tuple() == (256, 4, 4096, 3, 0, 1)
+    for esz, items in ((4, 8), (8, 8), (16, 4)):
+        assert lib.rz_dispatch_method_config(2, esz, ctypes.byref(out)) == 0
+        assert out.as_tuple() == (1024, items, 1024, 5, 11, 2)
+        assert method_config(2, esz).as_tuple() == out.as_tuple()
+    assert method_config(0, 4) is None and method_config(3, 4) is None
+    assert lib.rz_dispatch_method_config(0, 4, ctypes.byref(out)) == _native.RZ_EINVAL
+    assert lib.rz_dispatch_method_config(3, 4, ctypes.byref(out)) == _native.RZ_EINVAL
+    assert lib.rz_dispatch_method_config(2, 12, ctypes.byref(out)) == _native.RZ_EINVAL

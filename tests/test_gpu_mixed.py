@@ -67,3 +67,37 @@ def test_plain_rule_path_matches_oracle(ctx, oracle):
         else:
             want = expected_canonical(oracle, A0, R0)
             assert first_mismatch(got[: n - k], want[: n - k]) < 0
+
+
+def test_remove_auto_matches_oracle_and_dispatcher(ctx, oracle):
+    """rz_remove_auto: the method chosen in the library equals the Python
+    dispatcher's choice for the same list, and the result equals the oracle
+    (canonical Redzone, or the stable order for the partition).  Without
+    parameters it follows the k < n/2 rule."""
+    disp = D.Dispatcher.load(ctx)
+    ctx.set_dispatch_params(disp.params.cpu().numpy())
+    specs = MX.stream_specs(40, seed=13, log2n=(8, 18))
+    specs += [(5000, 4000, 4, "uniform"), (5000, 10, 8, "clustered")]
+    for i, spec in enumerate(specs):
+        n, k = spec[0], spec[1]
+        announce("auto", i, spec)
+        A, R = MX.build_list(ctx, spec, 1300 + 7 * i)
+        A0, R0 = A.cpu().numpy(), R.cpu().numpy()
+        want_m = int(disp.choose(disp.features([(A, R)]))[0]) if 0 < k < n else 0
+        new_len, m = ctx.remove_auto(A, R)
+        torch.cuda.synchronize()
+        assert new_len == n - k and m == want_m, (spec, m, want_m)
+        got = A.cpu().numpy()
+        if m == 3:
+            assert oracle.check(A0, got, R0, mode=1)[0] == 0
+        else:
+            want = expected_canonical(oracle, A0, R0)
+            assert first_mismatch(got[: n - k], want[: n - k]) < 0, (spec, m)
+    ctx.set_dispatch_params(None)
+    A, R = MX.build_list(ctx, (5000, 4000, 4, "uniform"), 77)
+    _, m = ctx.remove_auto(A, R)
+    assert m == 3
+    A, R = MX.build_list(ctx, (5000, 100, 4, "uniform"), 78)
+    _, m = ctx.remove_auto(A, R)
+    assert m == 0
+    ctx.set_dispatch_params(disp.params.cpu().numpy())

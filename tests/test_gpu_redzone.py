@@ -14,7 +14,7 @@ import torch
 
 import redzone_amd as rz
 from redzone_amd import workloads as W
-from tests.parity import announce, expected_canonical, gpu_remove_parity
+from tests.parity import announce, expected_canonical, first_mismatch, gpu_remove_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -354,3 +354,54 @@ def test_full_size_rec16(ctx, oracle, order):
     got = A.cpu().numpy()
     want = expected_canonical(oracle, A_np, R_np)
     assert np.array_equal(got[: n - k], want[: n - k])
+
+
+def _torch_tail_bits(R, n, k):
+    """The k-bit tail bitmap built with torch ops (distinct removals: summing the
+    bit values of a word is the same as or-ing them)."""
+    sz = n - k
+    z = (R[R >= sz] - sz).long()
+    words = torch.zeros((k + 31) // 32, dtype=torch.int64, device=R.device)
+    words.index_add_(0, z >> 5, torch.ones_like(z) << (z & 31))
+    return ((words + (1 << 31)) % (1 << 32) - (1 << 31)).to(torch.int32)
+
+
+@pytest.mark.parametrize("esz", [4, 16])
+def test_premarked(ctx, oracle, esz):
+    """Producer-side marks: rz_mark_removals equals the bitmap built with torch,
+    rz_remove_premarked gives the canonical array for every launch form (the
+    single-launch forms run as the fused form) and leaves the bitmap zero."""
+    for n, frac, pattern in [((1 << 18) + 5, 0.01, "uniform"), (1 << 18, 0.4, "tail_heavy"), (70000, 0.001, "uniform"),
+                             (1 << 20, 0.3, "clustered")]:
+        k = int(n * frac)
+        A = _payload(esz, n, seed=42)
+        R = W.removals(pattern, n, k, seed=43)
+        want = expected_canonical(oracle, A.numpy(), R.numpy())
+        dR = R.cuda()
+        ref_bits = _torch_tail_bits(dR, n, k)
+        bits = torch.zeros((k + 31) // 32, dtype=torch.int32, device="cuda")
+        announce("mark_removals", esz, n, k, pattern)
+        ctx.mark_removals(dR, n, bits)
+        torch.cuda.synchronize()
+        assert torch.equal(bits, ref_bits)
+        for cfg in (None, rz.TileConfig(256, 4, 64, 0, 0, 0), rz.TileConfig(256, 4, 4096, 3, 0, 0),
+                    rz.TileConfig(256, 8, 4096, 4, 0, 0), rz.TileConfig(512, 8, 1024, 5, 9, 0),
+                    rz.TileConfig(512, 4, 1024, 5, 9, 2)):
+            announce("premarked", esz, n, k, pattern, cfg)
+            bits.copy_(ref_bits)
+            dA = A.cuda()
+            dR = R.cuda()
+            assert ctx.remove_premarked(dA, dR, bits, config=cfg) == n - k
+            torch.cuda.synchronize()
+            got = dA.cpu().numpy()
+            m = first_mismatch(got[: n - k], want[: n - k])
+            assert m < 0, (cfg, m)
+            assert int(bits.abs().sum().item()) == 0, "the bitmap must be left zero"
+    announce("premarked k == n")
+    n = 4096
+    A = _payload(4, n, seed=44).cuda()
+    R = torch.randperm(n, dtype=torch.int64).to(torch.int32).cuda()
+    bits = _torch_tail_bits(R, n, n)
+    assert ctx.remove_premarked(A, R, bits) == 0
+    torch.cuda.synchronize()
+    assert int(bits.abs().sum().item()) == 0
