@@ -16,8 +16,16 @@ def _declared_functions():
             continue
         text = open(os.path.join(ROOT, "include", fname)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        # device-side inline helpers (redzone_device.h) are not library exports
+        text = re.sub(r"__device__[^{]*\{.*?\n\}", "", text, flags=re.S)
+        text = re.sub(r"#define[^\n]*", "", text)
         names.update(re.findall(r"\b(rz_[a-z0-9_]+)\s*\(", text))
     return names
+
+
+def test_device_header_declares_the_producer_helper():
+    text = open(os.path.join(ROOT, "include", "redzone_device.h")).read()
+    assert "__device__ inline void rz_mark_removal(" in text
 
 
 def test_library_exports_every_declared_symbol():
