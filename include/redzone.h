@@ -84,6 +84,22 @@ typedef struct rz_tile_config {
 rz_status rz_remove_premarked(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, int32_t* d_removals,
                               int64_t k, uint32_t* d_tail_bits, uint32_t flags, const rz_tile_config* cfg,
                               void* stream);
+/* Add and remove in one call (paper p.9 sec. 4.6: append G, then Redzone on
+ * A+G): the g items d_add are appended after the n items of d_data (which must
+ * hold capacity >= n + g items), then the k removals (indices into the
+ * combined list, usually < n) are removed as by rz_remove on n + g items.  The
+ * new length is n + g - k.  When g >= k and every removal is below n, the red
+ * zone lies inside the appended items, so every removal is a plain move of an
+ * added item into its hole. */
+rz_status rz_add_remove(rz_ctx* ctx, void* d_data, int64_t n, int64_t capacity, uint32_t elem_bytes,
+                        const void* d_add, int64_t g, int32_t* d_removals, int64_t k, uint32_t flags, void* stream);
+/* In-place gather by a preserve list (paper p.14, eq. 3): A'[i] = A[P[i]] for
+ * i < p, with P strictly ascending and P[p-1] < n (checked with RZ_VALIDATE).
+ * The survivors keep their order.  Tiles publish that they have read their
+ * items and write only after every earlier tile has, so no item is
+ * overwritten before it is read. */
+rz_status rz_gather_preserve(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, const int32_t* d_keep,
+                             int64_t p, uint32_t flags, void* stream);
 /* The mark step alone, into a caller-owned tail bitmap (zero beforehand),
  * e.g. on another stream while the list is still being written; built on
  * rz_mark_removal.  redzone.hxx:698-707 without the data sentinel. */

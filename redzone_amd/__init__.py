@@ -128,6 +128,35 @@ class Context:
                        ctypes.c_void_p(removals.data_ptr()), k, flags, ctypes.byref(config), s)
         return n - k
 
+    def add_remove(self, data, n, additions, removals, k=None, flags=0, stream=None):
+        """Appends `additions` after data[:n] (data must hold n + g items), then
+        removes `removals` (indices into the combined list) as remove() would on
+        n + g items.  Returns n + g - k."""
+        _check_tensor(data, "data", self.device)
+        _check_tensor(additions, "additions", self.device)
+        _check_tensor(removals, "removals", self.device)
+        import torch
+
+        if removals.dtype != torch.int32:
+            raise ValueError("removals must be int32")
+        esz = elem_bytes_of(data)
+        if elem_bytes_of(additions) != esz:
+            raise ValueError("additions must have the list's item size")
+        g = additions.shape[0]
+        k = removals.shape[0] if k is None else int(k)
+        self._call("rz_add_remove", self._h, ctypes.c_void_p(data.data_ptr()), int(n), data.shape[0], esz,
+                   ctypes.c_void_p(additions.data_ptr()), g, ctypes.c_void_p(removals.data_ptr()), k, flags,
+                   _stream_handle(stream, self.device))
+        return int(n) + g - k
+
+    def gather_preserve(self, data, keep, n=None, p=None, flags=0, stream=None):
+        """In place: data[i] = data[keep[i]] for i < p (keep strictly ascending,
+        int32).  Returns p."""
+        n, p, esz = self._args(data, keep, n, p)
+        self._call("rz_gather_preserve", self._h, ctypes.c_void_p(data.data_ptr()), n, esz,
+                   ctypes.c_void_p(keep.data_ptr()), p, flags, _stream_handle(stream, self.device))
+        return p
+
     def mark_removals(self, removals, n, tail_bits, k=None, stream=None):
         """The mark step alone into tail_bits (int32 device tensor of ceil(k/32)
         words, zero beforehand): bit r-(n-k) for every removal r >= n-k."""

@@ -123,6 +123,8 @@ SIGNATURES = [
     ("rz_mlp_adam", _i32, [_vp, _vp, _vp, _vp, _vp, _i32, ctypes.c_float, _vp]),
     ("rz_remove_premarked", _i32, [_vp, _vp, _i64, _u32, _vp, _i64, _vp, _u32, ctypes.POINTER(TileConfig), _vp]),
     ("rz_mark_removals", _i32, [_vp, _vp, _i64, _i64, _vp, _vp]),
+    ("rz_add_remove", _i32, [_vp, _vp, _i64, _i64, _u32, _vp, _i64, _vp, _i64, _u32, _vp]),
+    ("rz_gather_preserve", _i32, [_vp, _vp, _i64, _u32, _vp, _i64, _u32, _vp]),
     ("rz_dispatch_method_config", _i32, [_i32, _u32, ctypes.POINTER(TileConfig)]),
     ("rz_dispatch_set_params", _i32, [_vp, _vp]),
     ("rz_remove_auto", _i32, [_vp, _vp, _i64, _u32, _vp, _i64, _u32, ctypes.POINTER(_i32), _vp]),
