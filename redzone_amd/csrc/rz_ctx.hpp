@@ -21,9 +21,7 @@ enum CtrlWord : int {
   kCtrlValidate = 192,    // validation flags: 1 = out of range, 2 = duplicate, 4 = not ascending
   kCtrlMoveEpoch = 224,   // call counter of the fused move-and-pair form (bumped by its last workgroup)
   kCtrlMoveExit = 240,    // workgroups of the fused kernel that have finished (reset by the last)
-  kCtrlGatherEpoch = 256, // call counter of the preserve-list gather (advanced by its last workgroup)
-  kCtrlGatherExit = 272,  // workgroups of the gather that have finished (reset by the last)
-  kCtrlWords = 288
+  kCtrlWords = 256
 };
 
 enum ErrorBits : uint32_t {
@@ -66,7 +64,7 @@ struct rz_ctx {
   rz::DeviceBuffer dense_ent;    // dense form: coarse-bin head writes (positions, then values)
   rz::DeviceBuffer dense_fine;   // dense form: fine-bin head writes (fixed capacity per bin)
   rz::DeviceBuffer dense_src;    // dense form: source orphan values by rank
-  rz::DeviceBuffer gstatus;    // look-back status words of the preserve-list gather
+  rz::DeviceBuffer gstatus;    // preserve-list gather: the gathered items before they are copied back
   rz::DeviceBuffer dispatch;   // network parameters, one list descriptor, its features and the choice
   bool dispatch_ready = false; // parameters installed (else the k < n/2 rule)
   uint8_t* auto_host = nullptr;  // pinned staging of the descriptor and the choice

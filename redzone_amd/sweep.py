@@ -12,6 +12,11 @@ Config 3: n = 2^27 16-byte records {uint64 key, uint32 tag, float32 value},
 k = n/100 uniform removals, the removal list once randomly permuted and once
 sorted ascending (the sorted list is also timed with RZ_SORTED).
 
+Gather: n = 2^28 uint32, a sorted uniform preserve list of p = n * keep
+positions (keep in {0.001, 0.01, 0.1, 0.5, 0.99}) timed with the in-place
+gather, and its complement as a removal list timed with Redzone and the O(n)
+partition.
+
 Every call starts from pristine copies of the list and the removal list, with
 a 1 GiB read in between so that the caches hold no useful lines (cold
 timing); a point's time is the median over `reps` calls, from events around
@@ -109,9 +114,39 @@ def sweep_config3(ctx, rz, W, torch, reps, emit):
         emit(res)
 
 
+def sweep_gather(ctx, rz, W, torch, reps, emit):
+    """The preserve-list gather against Redzone and the O(n) partition on the
+    same split: P = a sorted uniform sample of p positions, R = its complement."""
+    n = 1 << 28
+    time_call = _timer(torch)
+    A0 = W.payload(4, n, seed=2026, device="cuda")
+    A = torch.empty_like(A0)
+    for keep in (0.001, 0.01, 0.1, 0.5, 0.99):
+        p = int(n * keep)
+        Rm = W.uniform_removals(n, n - p, seed=2030, device="cuda")  # the removals, unsorted
+        mask = torch.ones(n, dtype=torch.bool, device="cuda")
+        mask[Rm.long()] = False
+        P = torch.nonzero(mask).flatten().to(torch.int32)
+        del mask
+        R = torch.empty_like(Rm)
+
+        def restore():
+            A.copy_(A0)
+            R.copy_(Rm)
+
+        res = {"config": "gather", "n": n, "p": p, "keep": keep,
+               "gather_ms": round(time_call(restore, lambda: ctx.gather_preserve(A, P, n=n), reps), 4),
+               "redzone_ms": round(time_call(restore, lambda: ctx.remove(A, R), reps), 4),
+               "partition_ms": round(time_call(restore, lambda: ctx.partition(A, R), max(2, reps // 2)), 4)}
+        res["gather_GBps"] = round((4 * p + 4 * p + 4 * p) / (res["gather_ms"] * 1e-3) / 1e9, 1)
+        emit(res)
+        del Rm, R, P
+        torch.cuda.empty_cache()
+
+
 def main():
     p = argparse.ArgumentParser()
-    p.add_argument("--config", default="all", choices=["2", "3", "all"])
+    p.add_argument("--config", default="all", choices=["2", "3", "gather", "all"])
     p.add_argument("--reps", type=int, default=5)
     args = p.parse_args()
     import torch
@@ -128,6 +163,8 @@ def main():
         sweep_config2(ctx, rz, W, torch, args.reps, emit)
     if args.config in ("3", "all"):
         sweep_config3(ctx, rz, W, torch, args.reps, emit)
+    if args.config in ("gather", "all"):
+        sweep_gather(ctx, rz, W, torch, args.reps, emit)
 
 
 if __name__ == "__main__":
