@@ -61,15 +61,18 @@ typedef struct rz_ctx rz_ctx;
 typedef struct rz_tile_config {
   int32_t wg_size;     /* 256 or 512 */
   int32_t items;       /* items per thread per step: 2, 4, 8, 16 (variants 0/1); 4 or 8 (variant 2, batches) */
-  int32_t max_slices;  /* variants 0/1/3/4: upper bound on slices (workgroups of the move step) */
+  int32_t max_slices;  /* variants 0/1/3/4/6: upper bound on slices (workgroups of the move step) */
   int32_t variant;     /* 0 = three kernels on the stream (mark, move, pair), 1 = one cooperative kernel,
                           2 = persistent queue, 3 = one ordered launch (roles by workgroup id),
                           4 = two kernels (mark, then move with in-kernel orphan pairing; slices longer than
                               the workgroup LDS allows run as variant 0),
                           5 = dense form: head writes grouped by destination bin, then one scatter per bin
-                              (for k/n of several percent and up) */
+                              (for k/n of several percent and up),
+                          6 = variant 4 in one launch: mark workgroups first, move slices after them wait for
+                              the marks (same LDS limit and fallback as variant 4) */
   int32_t slice_steps; /* variant 2 and batches: steps per slice (0 = 4); variant 5: log2 of the target
-                          number of destination bins, 5..10 (0 = 9) */
+                          number of destination bins, 5..10 (0 = 9); variant 6: log2 of the removals
+                          each mark workgroup takes, 8..16 (0 = 12) */
   int32_t chunk;       /* variant 2 and batches: queue items taken per ticket (0 = 1) */
 } rz_tile_config;
 
@@ -78,7 +81,7 @@ typedef struct rz_tile_config {
  * ceil(k/32) words with bit r-(n-k) set for every removal r >= n-k and no
  * other bit; the call skips the mark step and leaves the bitmap zero.  cfg may
  * be null (the tuned table); shapes whose single launch includes the mark
- * (variants 1-3) run as variant 4.  RZ_SORTED is ignored and RZ_STATS counts
+ * (variants 1-3 and 6) run as variant 4.  RZ_SORTED is ignored and RZ_STATS counts
  * no tail removals (the marks were not made here).  Has no counterpart in the
  * reference: its phase 1 always scans the list (redzone.hxx:698-707). */
 rz_status rz_remove_premarked(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, int32_t* d_removals,

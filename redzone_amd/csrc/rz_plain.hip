@@ -116,12 +116,15 @@ __global__ __launch_bounds__(256) void rz_pair(T* __restrict__ A, const int32_t*
 // ---------------------------------------------------------------------------
 constexpr uint64_t kStAgg = 1ull << 62, kStIncl = 2ull << 62, kStCount = (1ull << 62) - 1;
 
-template <typename T, int WG, int IPT>
-__global__ __launch_bounds__(WG) void rz_move_pair(T* __restrict__ A, const int32_t* __restrict__ R, uint32_t k,
-                                                    uint32_t sZ, uint32_t L, uint32_t ns,
-                                                    uint32_t* __restrict__ bits, uint64_t* __restrict__ mstatus,
-                                                    uint64_t* __restrict__ slots, uint32_t slot_cap,
-                                                    uint32_t* __restrict__ ctrl, int stats) {
+// COH: the bitmap was marked in the same launch (rz_mark_move_pair): wait for
+// the nm mark workgroups after the first step's loads of R and the red zone,
+// then read bitmap words with coherent loads.
+template <typename T, int WG, int IPT, bool COH>
+__device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t* __restrict__ R, uint32_t k,
+                                               uint32_t sZ, uint32_t L, uint32_t ns, uint32_t* __restrict__ bits,
+                                               uint64_t* __restrict__ mstatus, uint64_t* __restrict__ slots,
+                                               uint32_t slot_cap, uint32_t* __restrict__ ctrl, int stats,
+                                               uint32_t slice, uint32_t nm, uint32_t backoff) {
   constexpr int NW = WG / kWave;
   constexpr int E = MoveShared<WG, IPT>::E;
   constexpr int PER = (E + kWave - 1) / kWave;
@@ -130,7 +133,6 @@ __global__ __launch_bounds__(WG) void rz_move_pair(T* __restrict__ A, const int3
   extern __shared__ uint32_t s_orph[];  // L entries
   uint32_t* s_off = sh.off;
   const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
-  const uint32_t slice = blockIdx.x;
   const uint32_t a = slice * L, b = min(a + L, k);
   T* __restrict__ Z = A + sZ;
   const uint32_t epoch = ld_relaxed(&ctrl[kCtrlMoveEpoch]);
@@ -150,10 +152,18 @@ __global__ __launch_bounds__(WG) void rz_move_pair(T* __restrict__ A, const int3
       if (p < b) {
         d[j] = (uint32_t)R[p];
         z[j] = Z[p];
-        wv[j] = bits[p >> 5];
+        if constexpr (!COH) wv[j] = bits[p >> 5];
       } else {
         d[j] = sZ;           // invalid destination
         wv[j] = 0xffffffffu; // dead source
+      }
+    }
+    if constexpr (COH) {
+      if (base == a) wait_count_backoff(&ctrl[kCtrlMarkDone], nm, ctrl, backoff);  // every mark of this launch is in
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint32_t p = base + j * WG + tid;
+        if (p < b) wv[j] = ld_relaxed(&bits[p >> 5]);
       }
     }
 #pragma unroll
@@ -280,8 +290,48 @@ __global__ __launch_bounds__(WG) void rz_move_pair(T* __restrict__ A, const int3
   // starts the next call's epoch
   if (tid == 0 && add_relaxed(&ctrl[kCtrlMoveExit], 1u) == ns - 1u) {
     st_relaxed(&ctrl[kCtrlMoveExit], 0u);
+    if constexpr (COH) st_relaxed(&ctrl[kCtrlMarkDone], 0u);  // every slice has passed its wait
     st_relaxed(&ctrl[kCtrlMoveEpoch], epoch + 1u);
   }
+}
+
+template <typename T, int WG, int IPT>
+__global__ __launch_bounds__(WG) void rz_move_pair(T* __restrict__ A, const int32_t* __restrict__ R, uint32_t k,
+                                                    uint32_t sZ, uint32_t L, uint32_t ns,
+                                                    uint32_t* __restrict__ bits, uint64_t* __restrict__ mstatus,
+                                                    uint64_t* __restrict__ slots, uint32_t slot_cap,
+                                                    uint32_t* __restrict__ ctrl, int stats) {
+  move_pair_body<T, WG, IPT, false>(A, R, k, sZ, L, ns, bits, mstatus, slots, slot_cap, ctrl, stats, blockIdx.x, 0u, 0u);
+}
+
+// Variant 6: the mark step in the same launch.  Workgroups [0, nm) each mark
+// a tile of `tile` removals and count themselves done; the move-and-pair
+// slices come after them in dispatch order (so every mark workgroup is running
+// or done when a slice waits), prefetch their first step and wait for the count.
+constexpr int kFusedMarkTileLog = 12;  // default tile: 4096 removals
+constexpr uint32_t kMarkWaitBackoff = 16;  // longest pause between polls of the mark count: 16 x 64 cycles
+
+template <typename T, int WG, int IPT>
+__global__ __launch_bounds__(WG) void rz_mark_move_pair(T* __restrict__ A, const int32_t* __restrict__ R, uint32_t k,
+                                                         uint32_t sZ, uint32_t L, uint32_t ns,
+                                                         uint32_t* __restrict__ bits,
+                                                         uint64_t* __restrict__ mstatus,
+                                                         uint64_t* __restrict__ slots, uint32_t slot_cap,
+                                                         uint32_t* __restrict__ ctrl, int stats, uint32_t nm,
+                                                         uint32_t tile, uint32_t backoff, int sorted) {
+  if (blockIdx.x < nm) {
+    const uint32_t lo = blockIdx.x * tile, hi = min(lo + tile, k);
+    if (!sorted || (uint32_t)R[hi - 1] >= sZ) {  // a sorted chunk below the red zone has no marks
+      if (stats)
+        mark_body<true>(R + lo, hi - lo, k, sZ, bits, (uint64_t*)&ctrl[kCtrlStatTail], threadIdx.x, WG);
+      else
+        mark_body<false>(R + lo, hi - lo, k, sZ, bits, nullptr, threadIdx.x, WG);
+    }
+    signal_done(&ctrl[kCtrlMarkDone]);
+    return;
+  }
+  move_pair_body<T, WG, IPT, true>(A, R, k, sZ, L, ns, bits, mstatus, slots, slot_cap, ctrl, stats, blockIdx.x - nm,
+                                   nm, backoff);
 }
 
 // ---------------------------------------------------------------------------
@@ -445,6 +495,28 @@ static bool dispatch_move_pair(int wg, int ipt, hipStream_t s, T* A, const int32
   return false;
 }
 
+// Variant 6: nm mark workgroups, then the ns move-and-pair slices, one grid.
+template <typename T>
+static bool dispatch_mark_move_pair(int wg, int ipt, hipStream_t s, T* A, const int32_t* R, uint32_t k, uint32_t sZ,
+                                    uint32_t L, uint32_t ns, uint32_t* bits, uint64_t* mstatus, uint64_t* slots,
+                                    uint32_t slot_cap, uint32_t* ctrl, int stats, int tile_log, int sorted) {
+  const uint32_t bo = kMarkWaitBackoff;
+  const size_t lds = (size_t)L * 4;
+  tile_log = tile_log == 0 ? kFusedMarkTileLog : (tile_log < 8 ? 8 : (tile_log > 16 ? 16 : tile_log));
+  const uint32_t tile = 1u << tile_log;
+  const uint32_t nm = (k + tile - 1) / tile;
+#define RZ_MMP_CASE(W, I)                                                                                      \
+  if (wg == W && ipt == I) {                                                                                   \
+    hipLaunchKernelGGL((rz_mark_move_pair<T, W, I>), dim3(nm + ns), dim3(W), lds, s, A, R, k, sZ, L, ns, bits, \
+                       mstatus, slots, slot_cap, ctrl, stats, nm, tile, bo, sorted);                            \
+    return true;                                                                                               \
+  }
+  RZ_MMP_CASE(256, 2) RZ_MMP_CASE(256, 4) RZ_MMP_CASE(256, 8) RZ_MMP_CASE(256, 16)
+  RZ_MMP_CASE(512, 2) RZ_MMP_CASE(512, 4) RZ_MMP_CASE(512, 8) RZ_MMP_CASE(512, 16)
+#undef RZ_MMP_CASE
+  return false;
+}
+
 // The fused form keeps a slice's orphan list in LDS (4 bytes per pair); a
 // slice too long for the device's per-workgroup LDS runs the three-kernel form.
 static bool fused_fits(const rz_ctx* ctx, const rz_tile_config& cfg, uint32_t L) {
@@ -524,7 +596,8 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
     if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;
     return prof_mark(ctx, flags, 3, s);
   }
-  const bool fused = cfg.variant == 4 && fused_fits(ctx, cfg, L);
+  const bool fused = (cfg.variant == 4 || cfg.variant == 6) && fused_fits(ctx, cfg, L);
+  const bool one_launch = fused && cfg.variant == 6;  // marks inside the fused kernel
   if (fused) {
     if ((st = ensure(ctx, ctx->mstatus, 2 * (size_t)kMaxSlices * 8, s)) != RZ_OK) return st;
     if ((st = ensure(ctx, ctx->slots, ((size_t)k / 2 + 1) * 8, s)) != RZ_OK) return st;
@@ -533,8 +606,8 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
   const bool dense_marks = !premarked && dense_bucket_marks(cfg, k, flags);
   const int64_t g1w = (k + 256 * 4 - 1) / (256 * 4);
   const dim3 g1((uint32_t)(g1w < (int64_t)ctx->cu_count * 8 ? g1w : ctx->cu_count * 8));
-  if (premarked || dense_marks) {
-    // the caller's producer set the tail marks, or the dense form sets them itself
+  if (premarked || dense_marks || one_launch) {
+    // the caller's producer set the tail marks, or the launch below sets them itself
   } else if (stats)
     hipLaunchKernelGGL(rz_mark_tail<true>, g1, dim3(256), 0, s, R, (uint32_t)k, sZ, bits,
                        (uint64_t*)(ctrl + kCtrlStatTail));
@@ -547,6 +620,15 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
   if ((st = prof_mark(ctx, flags, 1, s)) != RZ_OK) return st;
   if (cfg.variant == 5) {
     if ((st = dense_moves<T>(ctx, A, n, R, k, cfg, bits, ctrl, stats, dense_marks, s)) != RZ_OK) return st;
+    if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;
+    return prof_mark(ctx, flags, 3, s);
+  }
+  if (one_launch) {
+    if (!dispatch_mark_move_pair<T>(cfg.wg_size, cfg.items, s, A, R, (uint32_t)k, sZ, L, ns, bits,
+                                    (uint64_t*)ctx->mstatus.ptr, (uint64_t*)ctx->slots.ptr, (uint32_t)(k / 2 + 1),
+                                    ctrl, stats, cfg.slice_steps, (flags & RZ_SORTED) ? 1 : 0))
+      return RZ_EINVAL;
+    RZ_HIP(hipGetLastError());
     if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;
     return prof_mark(ctx, flags, 3, s);
   }
@@ -575,7 +657,7 @@ rz_status launch_plain(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, int32_t
   const bool premarked = (flags & kFlagPremarked) != 0;
   // the single-launch forms include their own mark step: with the marks set
   // by the caller, run their shape as the two-kernel fused form
-  if (premarked && cfg.variant >= 1 && cfg.variant <= 3) cfg.variant = 4;
+  if (premarked && ((cfg.variant >= 1 && cfg.variant <= 3) || cfg.variant == 6)) cfg.variant = 4;
   if (cfg.variant == 2) return launch_queue_single(ctx, data, n, esz, R, k, flags, cfg, s);
   const bool coop = !premarked && (cfg.variant == 1 || (flags & RZ_COOPERATIVE));
   switch (esz) {

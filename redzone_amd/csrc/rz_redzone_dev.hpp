@@ -297,4 +297,22 @@ __device__ __forceinline__ void wait_count(const uint32_t* counter, uint32_t tar
   __syncthreads();
 }
 
+// As wait_count, with a pause that doubles from 64 to 64*cap cycles between
+// polls, so that many waiting workgroups do not crowd the counter's L2 channel.
+__device__ __forceinline__ void wait_count_backoff(const uint32_t* counter, uint32_t target, uint32_t* ctrl,
+                                                   uint32_t cap) {
+  if (threadIdx.x == 0) {
+    uint32_t spins = 0, pause = 1;
+    while (ld_relaxed(counter) < target) {
+      for (uint32_t i = 0; i < pause; ++i) __builtin_amdgcn_s_sleep(1);
+      if (pause < cap) pause <<= 1;
+      if (++spins > (1u << 22)) {
+        or_relaxed(&ctrl[kCtrlError], kErrBarrierTimeout);
+        break;
+      }
+    }
+  }
+  __syncthreads();
+}
+
 }  // namespace rz

@@ -109,12 +109,15 @@ def test_ordered_variant(ctx, oracle, esz):
             gpu_remove_parity(ctx, oracle, A, R, config=cfg, want=want)
 
 
+@pytest.mark.parametrize("variant", [4, 6])
 @pytest.mark.parametrize("esz", [4, 8, 16])
-def test_fused_variant(ctx, oracle, esz):
+def test_fused_variant(ctx, oracle, esz, variant):
     """Mark, then one move kernel that pairs orphans through look-back and
-    exchange words: the same array as the three-kernel form.  Consecutive calls
-    alternate the look-back status halves, and the slice count changes between
-    calls, so a half left dirty would show up here."""
+    exchange words (variant 4), or both in one launch with the mark workgroups
+    first (variant 6): the same array as the three-kernel form.  Consecutive
+    calls alternate the look-back status halves, and the slice count changes
+    between calls, so a half left dirty (or a mark counter not reset) would
+    show up here."""
     cases = [((1 << 18) + 5, 0.01, 4096, "uniform"), ((1 << 18) + 5, 0.4, 64, "tail_heavy"),
              (1 << 16, 0.9, 7, "clustered"), (1 << 20, 0.05, 4096, "tail_heavy"), (3000, 0.5, 4096, "uniform"),
              (70000, 0.001, 1, "uniform"), ((1 << 20) + 77, 0.3, 4096, "uniform"), (1 << 21, 0.5, 512, "clustered")]
@@ -124,19 +127,20 @@ def test_fused_variant(ctx, oracle, esz):
         R = W.removals(pattern, n, k, seed=37)
         want = expected_canonical(oracle, A.numpy(), R.numpy())
         for wg, it in ((256, 2), (256, 4), (256, 16), (512, 8)):
-            cfg = rz.TileConfig(wg, it, ms, 4, 0, 0)
+            cfg = rz.TileConfig(wg, it, ms, variant, 0, 0)
             announce("fused", esz, n, k, pattern, cfg)
             gpu_remove_parity(ctx, oracle, A, R, config=cfg, want=want)
 
 
-def test_fused_variant_flags(ctx, oracle):
-    """RZ_STATS, RZ_PRESERVE_REMOVALS and RZ_SORTED on the fused form; and a
+@pytest.mark.parametrize("variant", [4, 6])
+def test_fused_variant_flags(ctx, oracle, variant):
+    """RZ_STATS, RZ_PRESERVE_REMOVALS and RZ_SORTED on the fused forms; and a
     slice too long for LDS (runs as the three-kernel form)."""
     n = 1 << 20
     k = n // 10
     A = W.index_payload(n)
     R = W.tail_heavy_removals(n, k, seed=38)
-    cfg = rz.TileConfig(256, 4, 4096, 4, 0, 0)
+    cfg = rz.TileConfig(256, 4, 4096, variant, 0, 0)
     announce("fused stats")
     _, dR = gpu_remove_parity(ctx, oracle, A, R, config=cfg, flags=rz.RZ_STATS | rz.RZ_PRESERVE_REMOVALS)
     s = ctx.last_stats()
@@ -152,7 +156,7 @@ def test_fused_variant_flags(ctx, oracle):
     Rs = torch.sort(R).values.contiguous()
     gpu_remove_parity(ctx, oracle, A, Rs, config=cfg, flags=rz.RZ_SORTED)
     announce("fused long slices")
-    gpu_remove_parity(ctx, oracle, A, R, config=rz.TileConfig(256, 4, 1, 4, 0, 0))
+    gpu_remove_parity(ctx, oracle, A, R, config=rz.TileConfig(256, 4, 1, variant, 0, 0))
 
 
 @pytest.mark.parametrize("esz", [4, 8, 16])
@@ -385,7 +389,8 @@ def test_premarked(ctx, oracle, esz):
         torch.cuda.synchronize()
         assert torch.equal(bits, ref_bits)
         for cfg in (None, rz.TileConfig(256, 4, 64, 0, 0, 0), rz.TileConfig(256, 4, 4096, 3, 0, 0),
-                    rz.TileConfig(256, 8, 4096, 4, 0, 0), rz.TileConfig(512, 8, 1024, 5, 9, 0),
+                    rz.TileConfig(256, 8, 4096, 4, 0, 0), rz.TileConfig(256, 8, 4096, 6, 0, 0),
+                    rz.TileConfig(512, 8, 1024, 5, 9, 0),
                     rz.TileConfig(512, 4, 1024, 5, 9, 2)):
             announce("premarked", esz, n, k, pattern, cfg)
             bits.copy_(ref_bits)
