@@ -441,8 +441,10 @@ __global__ __launch_bounds__(1024) void rz_dense_scatter(T* __restrict__ A, cons
 
 // ---- bucketed tail marks ------------------------------------------------------
 template <int WG, int IPT>
+// k removals; marked positions [sZ, sZ + m) (m = k for the red zone)
 __global__ __launch_bounds__(WG) void rz_dense_tail_count(const int32_t* __restrict__ R, uint32_t k, uint32_t sZ,
-                                                           uint32_t L, uint32_t NBt, uint32_t* __restrict__ cnt) {
+                                                           uint32_t m, uint32_t L, uint32_t NBt,
+                                                           uint32_t* __restrict__ cnt) {
   __shared__ uint32_t hist[kCoarseMax];
   const uint32_t tid = threadIdx.x, j = blockIdx.x;
   const uint32_t a = j * L, b = min(a + L, k);
@@ -453,11 +455,11 @@ __global__ __launch_bounds__(WG) void rz_dense_tail_count(const int32_t* __restr
 #pragma unroll
     for (int q = 0; q < IPT; ++q) {
       const uint32_t p = base + q * WG + tid;
-      z[q] = p < b ? (uint32_t)R[p] - sZ : 0xffffffffu;  // wraps to >= k below the red zone
+      z[q] = p < b ? (uint32_t)R[p] - sZ : 0xffffffffu;  // wraps to >= m below the window
     }
 #pragma unroll
     for (int q = 0; q < IPT; ++q)
-      if (z[q] < k) atomicAdd(&hist[z[q] >> kTailBinLog], 1u);
+      if (z[q] < m) atomicAdd(&hist[z[q] >> kTailBinLog], 1u);
   }
   __syncthreads();
   for (uint32_t q = tid; q < NBt; q += WG) cnt[(size_t)j * NBt + q] = hist[q];
@@ -466,7 +468,7 @@ __global__ __launch_bounds__(WG) void rz_dense_tail_count(const int32_t* __restr
 // Dynamic LDS: S red-zone offsets and S bin ids (S = WG * IPT).
 template <int WG, int IPT>
 __global__ __launch_bounds__(WG) void rz_dense_tail_entries(const int32_t* __restrict__ R, uint32_t k, uint32_t sZ,
-                                                             uint32_t L, uint32_t NBt,
+                                                             uint32_t m, uint32_t L, uint32_t NBt,
                                                              const uint32_t* __restrict__ off,
                                                              const uint32_t* __restrict__ binbase,
                                                              uint32_t* __restrict__ ent) {
@@ -491,7 +493,7 @@ __global__ __launch_bounds__(WG) void rz_dense_tail_entries(const int32_t* __res
       z[q] = p < b ? (uint32_t)R[p] - sZ : 0xffffffffu;
     }
 #pragma unroll
-    for (int q = 0; q < IPT; ++q) rk[q] = z[q] < k ? atomicAdd(&lcnt[z[q] >> kTailBinLog], 1u) : 0u;
+    for (int q = 0; q < IPT; ++q) rk[q] = z[q] < m ? atomicAdd(&lcnt[z[q] >> kTailBinLog], 1u) : 0u;
     __syncthreads();
     for (uint32_t q = tid; q < NBt; q += WG) {
       gb[q] = gcur[q];
@@ -501,7 +503,7 @@ __global__ __launch_bounds__(WG) void rz_dense_tail_entries(const int32_t* __res
     const uint32_t total = lds_excl_scan<WG>(lcnt, NBt, tmp);
 #pragma unroll
     for (int q = 0; q < IPT; ++q) {
-      if (z[q] >= k) continue;
+      if (z[q] >= m) continue;
       const uint32_t bin = z[q] >> kTailBinLog;
       const uint32_t e = lcnt[bin] + rk[q];
       sz[e] = z[q];
@@ -519,10 +521,10 @@ __global__ __launch_bounds__(WG) void rz_dense_tail_entries(const int32_t* __res
 }
 
 // One workgroup per tail bin: set the bin's bits in a dynamic-LDS bitmap of
-// 2^kTailBinLog bits, then write its words whole (every word of the k-bit
+// 2^kTailBinLog bits, then write its words whole (every word of the m-bit
 // bitmap belongs to exactly one bin).
 __global__ __launch_bounds__(1024) void rz_dense_tail_mark(const uint32_t* __restrict__ ent,
-                                                          const uint32_t* __restrict__ binbase, uint32_t k,
+                                                          const uint32_t* __restrict__ binbase, uint32_t m,
                                                           uint32_t* __restrict__ bits,
                                                           uint64_t* __restrict__ stat_tail, uint32_t NBt) {
   constexpr uint32_t W = 1u << (kTailBinLog - 5);
@@ -538,7 +540,7 @@ __global__ __launch_bounds__(1024) void rz_dense_tail_mark(const uint32_t* __res
     if (r < (1u << kTailBinLog)) atomicOr(&lb[r >> 5], 1u << (r & 31));
   }
   __syncthreads();
-  const uint32_t nw = (k + 31u) >> 5;
+  const uint32_t nw = (uint32_t)(((uint64_t)m + 31u) >> 5);
   for (uint32_t w = tid; w < W; w += 1024u) {
     const uint32_t gw = (z0 >> 5) + w;
     if (gw < nw) bits[gw] = lb[w];
@@ -605,12 +607,12 @@ static rz_status dense_launch(rz_ctx* ctx, T* A, const int32_t* R, uint32_t k, u
   if (d.NBt) {
     // the red-zone offsets use ent_d's space before the head writes do
     uint32_t* toff = cnt + (size_t)d.nt * d.NBt;
-    hipLaunchKernelGGL((rz_dense_tail_count<WG, IPT>), dim3(d.nt), dim3(WG), 0, s, R, k, sZ, d.L, d.NBt, cnt);
+    hipLaunchKernelGGL((rz_dense_tail_count<WG, IPT>), dim3(d.nt), dim3(WG), 0, s, R, k, sZ, k, d.L, d.NBt, cnt);
     hipLaunchKernelGGL(rz_dense_colscan, dim3(d.NBt / 32), dim3(1024), 0, s, cnt, d.nt, d.NBt, toff, tbtot);
     hipLaunchKernelGGL(rz_dense_topscan, dim3(1), dim3(1024), 0, s, tbtot, d.NBt, tbinbase, ocnt, 0u,
                        (uint64_t*)nullptr, ctrl, 0);
-    hipLaunchKernelGGL((rz_dense_tail_entries<WG, IPT>), dim3(d.nt), dim3(WG), lds_tail, s, R, k, sZ, d.L, d.NBt,
-                       toff, tbinbase, ent_d);
+    hipLaunchKernelGGL((rz_dense_tail_entries<WG, IPT>), dim3(d.nt), dim3(WG), lds_tail, s, R, k, sZ, k, d.L,
+                       d.NBt, toff, tbinbase, ent_d);
     hipLaunchKernelGGL(rz_dense_tail_mark, dim3(d.NBt), dim3(1024), lds_mark, s, ent_d, tbinbase, k, bits,
                        stats ? (uint64_t*)(ctrl + kCtrlStatTail) : (uint64_t*)nullptr, d.NBt);
   }
@@ -626,6 +628,56 @@ static rz_status dense_launch(rz_ctx* ctx, T* A, const int32_t* R, uint32_t k, u
                      binbase + d.NB, fent_d, fent_v, fcur, d.NBf, d.bsf, ctrl);
   hipLaunchKernelGGL(rz_dense_scatter<T>, dim3(d.NBf * d.G), dim3(1024), 0, s, A, fent_d, fent_v, fcur, d.NBf, d.bsf,
                      d.G, src, src_cap, sZ);
+  RZ_HIP(hipGetLastError());
+  return RZ_OK;
+}
+
+// Bucketed marks of the window [base, base + m): the k removals are grouped by
+// bin of 2^kTailBinLog positions (count, scan, staged entries), then one
+// workgroup per bin sets its bits in LDS and writes its words whole.  Every
+// word of the m-bit bitmap is written (zero where nothing is removed), so
+// the bitmap need not be clear beforehand.  Removals outside the window are
+// ignored.  Needs m <= kCoarseMax * 2^kTailBinLog.
+bool bucket_marks_fit(int64_t m) { return m > 0 && m <= ((int64_t)kCoarseMax << kTailBinLog); }
+
+rz_status bucket_marks(rz_ctx* ctx, const int32_t* R, int64_t k, uint32_t base, int64_t m, uint32_t* bits,
+                       hipStream_t s) {
+  constexpr int WG = 1024, IPT = 8;
+  constexpr uint32_t S = WG * IPT;
+  if (!bucket_marks_fit(m) || k <= 0) return RZ_EINVAL;
+  rz_tile_config cfg{WG, IPT, 1024, 5, 0, 2};
+  uint32_t L, nt;
+  slice_shape(k, cfg, kMaxSlices, &L, &nt);
+  uint32_t NBt = (uint32_t)((m + ((int64_t)1 << kTailBinLog) - 1) >> kTailBinLog);
+  NBt = (NBt + 31u) & ~31u;
+  rz_status st;
+  if ((st = ensure(ctx, ctx->dense_cnt, (size_t)nt * NBt * 4 * 2, s)) != RZ_OK) return st;
+  if ((st = ensure(ctx, ctx->dense_small, kSmallBytes, s)) != RZ_OK) return st;
+  if ((st = ensure(ctx, ctx->dense_ent, (size_t)k * 4, s)) != RZ_OK) return st;
+  const size_t lds_tail = (size_t)S * (4 + 2);
+  const size_t lds_mark = (size_t)4 << (kTailBinLog - 5);
+  static bool attr_set = false;
+  if (!attr_set) {
+    RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_tail_entries<WG, IPT>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_tail));
+    RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_tail_mark, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_mark));
+    attr_set = true;
+  }
+  uint32_t* cnt = (uint32_t*)ctx->dense_cnt.ptr;
+  uint32_t* toff = cnt + (size_t)nt * NBt;
+  uint32_t* tbtot = (uint32_t*)ctx->dense_small.ptr + 2 * kSmallStride;
+  uint32_t* tbinbase = tbtot + kSmallStride;
+  uint32_t* ent = (uint32_t*)ctx->dense_ent.ptr;
+  hipLaunchKernelGGL((rz_dense_tail_count<WG, IPT>), dim3(nt), dim3(WG), 0, s, R, (uint32_t)k, base, (uint32_t)m, L,
+                     NBt, cnt);
+  hipLaunchKernelGGL(rz_dense_colscan, dim3(NBt / 32), dim3(1024), 0, s, cnt, nt, NBt, toff, tbtot);
+  hipLaunchKernelGGL(rz_dense_topscan, dim3(1), dim3(1024), 0, s, tbtot, NBt, tbinbase, (const uint64_t*)nullptr, 0u,
+                     (uint64_t*)nullptr, (uint32_t*)nullptr, 0);
+  hipLaunchKernelGGL((rz_dense_tail_entries<WG, IPT>), dim3(nt), dim3(WG), lds_tail, s, R, (uint32_t)k, base,
+                     (uint32_t)m, L, NBt, toff, tbinbase, ent);
+  hipLaunchKernelGGL(rz_dense_tail_mark, dim3(NBt), dim3(1024), lds_mark, s, ent, tbinbase, (uint32_t)m, bits,
+                     (uint64_t*)nullptr, NBt);
   RZ_HIP(hipGetLastError());
   return RZ_OK;
 }

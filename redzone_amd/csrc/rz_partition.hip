@@ -6,7 +6,10 @@
 // (StreamCompact_OneBlock, redzone.hxx:526-603) or with thrust::remove
 // (Thrust_remove.hxx:15-17).  Here:
 //   rz_mark_all   sets one bit per removal in an n-bit bitmap (payload
-//                 agnostic) and bumps the look-back epoch;
+//                 agnostic) and bumps the look-back epoch; lists of 2^23
+//                 items and more with 2^20 removals and more group the
+//                 marks by bitmap region first (bucket_marks, rz_dense.hip)
+//                 and write every bitmap word whole;
 //   rz_compact    one pass over all n items in tiles of WG*IPT, tile id =
 //                 workgroup id (workgroups are dispatched in id order).  Each
 //                 tile ranks its kept items with wave ballots and one LDS scan,
@@ -17,6 +20,9 @@
 //                 end of its own input, so the in-place pass never overwrites
 //                 an item before it was read.  It clears the bitmap words it
 //                 read, so the bitmap is zero again for the next call.
+//                 The 16-byte aligned form stages the kept items in LDS and
+//                 writes them as whole vectors, and waits on the nearest
+//                 predecessor's status word before it scans a window.
 #include "rz_ctx.hpp"
 #include "rz_device.hpp"
 
@@ -137,25 +143,74 @@ __global__ __launch_bounds__(WG) void rz_compact(T* __restrict__ A, uint32_t n, 
 }
 
 // ---------------------------------------------------------------------------
-// Vectorised form: every thread moves whole 16-byte vectors (V = 16 / sizeof(T)
+// Vectorised form: every thread loads whole 16-byte vectors (V = 16 / sizeof(T)
 // items), the tile's bitmap words are read once into LDS (and cleared at
 // once), and a lane's rank among the kept items of a wave comes from three
-// bit-sliced ballots of its per-vector keep count.  Needs a 16-byte aligned list.
+// bit-sliced ballots of its per-vector keep count.  The kept items are staged
+// in LDS in tile order while wave 0 looks back, then written out as aligned
+// 16-byte vectors (single items only at the two ends).  Needs a 16-byte
+// aligned list.
 // ---------------------------------------------------------------------------
-template <int WG, int VPT>
-struct CompactShared {
-  static constexpr int NW = WG / kWave;
-  uint32_t off[VPT * NW + 1];
-  uint32_t bits[WG * VPT * 4 / 32 + 1];  // up to 4 items per vector
-  uint32_t excl;
-};
-
 __device__ __forceinline__ uint32_t keep_count_prefix(uint32_t c, uint32_t* total) {
   // c in [0, 4]: three bit planes
   const uint64_t b0 = ballot(c & 1u), b1 = ballot(c & 2u), b2 = ballot(c & 4u);
   *total = (uint32_t)__popcll(b0) + 2u * (uint32_t)__popcll(b1) + 4u * (uint32_t)__popcll(b2);
   return lane_rank(b0) + 2u * lane_rank(b1) + 4u * lane_rank(b2);
 }
+
+// item q of a 16-byte vector, and a vector of V items
+template <typename T, int Q> __device__ __forceinline__ T vec_item(const uint4& v) {
+  if constexpr (sizeof(T) == 4) {
+    return Q == 0 ? v.x : Q == 1 ? v.y : Q == 2 ? v.z : v.w;
+  } else if constexpr (sizeof(T) == 8) {
+    return Q == 0 ? ((uint64_t)v.y << 32 | v.x) : ((uint64_t)v.w << 32 | v.z);
+  } else {
+    return v;
+  }
+}
+template <typename T, int Q> __device__ __forceinline__ void vec_set(uint4& v, const T& e) {
+  if constexpr (sizeof(T) == 4) {
+    if (Q == 0) v.x = e; else if (Q == 1) v.y = e; else if (Q == 2) v.z = e; else v.w = e;
+  } else if constexpr (sizeof(T) == 8) {
+    if (Q == 0) { v.x = (uint32_t)e; v.y = (uint32_t)(e >> 32); } else { v.z = (uint32_t)e; v.w = (uint32_t)(e >> 32); }
+  } else {
+    v = e;
+  }
+}
+
+template <typename T, int V, int Q = 0>
+__device__ __forceinline__ void load_partial(const T* __restrict__ A, uint32_t p0, uint32_t n, uint4& v) {
+  if constexpr (Q < V) {
+    if (p0 + Q < n) vec_set<T, Q>(v, A[p0 + Q]);
+    load_partial<T, V, Q + 1>(A, p0, n, v);
+  }
+}
+
+template <typename T, int V, int Q = 0>
+__device__ __forceinline__ void stage_kept(T* __restrict__ stage, const uint4& v, uint32_t km, uint32_t pos) {
+  if constexpr (Q < V) {
+    if ((km >> Q) & 1u) stage[pos++] = vec_item<T, Q>(v);
+    stage_kept<T, V, Q + 1>(stage, v, km, pos);
+  }
+}
+
+template <typename T, int V, int Q = 0>
+__device__ __forceinline__ void gather_vec(const T* __restrict__ stage, uint32_t o, uint4& v) {
+  if constexpr (Q < V) {
+    vec_set<T, Q>(v, stage[o + Q]);
+    gather_vec<T, V, Q + 1>(stage, o, v);
+  }
+}
+
+template <typename T, int WG, int VPT>
+struct CompactShared {
+  static constexpr int NW = WG / kWave;
+  static constexpr uint32_t TILE = WG * VPT * (16 / sizeof(T));
+  T stage[TILE];
+  uint32_t bits[TILE / 32];
+  uint32_t off[VPT * NW + 1];
+  uint32_t excl;
+};
 
 template <typename T, int WG, int VPT>
 __global__ __launch_bounds__(WG) void rz_compact_vec(T* __restrict__ A, uint32_t n, uint32_t* __restrict__ bits,
@@ -167,7 +222,7 @@ __global__ __launch_bounds__(WG) void rz_compact_vec(T* __restrict__ A, uint32_t
   constexpr uint32_t TILE = WG * VPT * V;
   constexpr int WORDS = TILE / 32;
   static_assert(TILE % 32 == 0, "tiles are bitmap-word aligned");
-  __shared__ CompactShared<WG, VPT> sh;
+  __shared__ CompactShared<T, WG, VPT> sh;
   const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
   const uint32_t t = blockIdx.x;
   const uint32_t base = t * TILE;
@@ -184,35 +239,25 @@ __global__ __launch_bounds__(WG) void rz_compact_vec(T* __restrict__ A, uint32_t
     }
     sh.bits[w] = v;
   }
-  union Vec {
-    uint4 u;
-    T e[V];
-  };
-  Vec x[VPT];
+  uint4 x[VPT];
 #pragma unroll
   for (int j = 0; j < VPT; ++j) {
     const uint32_t p0 = base + (uint32_t)(j * WG + tid) * V;
     if (p0 + V <= n) {
-      x[j].u = *(const uint4*)(A + p0);
+      x[j] = *(const uint4*)(A + p0);
     } else {
-#pragma unroll
-      for (int q = 0; q < V; ++q)
-        if (p0 + q < n) x[j].e[q] = A[p0 + q];
+      x[j] = make_uint4(0u, 0u, 0u, 0u);
+      load_partial<T, V>(A, p0, n, x[j]);
     }
   }
   __syncthreads();
   uint32_t keep[VPT], pre[VPT];
 #pragma unroll
   for (int j = 0; j < VPT; ++j) {
-    const uint32_t r0 = (uint32_t)(j * WG + tid) * V;  // offset in the tile
+    const uint32_t r0 = (uint32_t)(j * WG + tid) * V;  // offset in the tile (V items share one bitmap word)
     const uint32_t p0 = base + r0;
-    uint32_t km = 0;
-#pragma unroll
-    for (int q = 0; q < V; ++q) {
-      const uint32_t r = r0 + q;
-      const bool dead = (sh.bits[r >> 5] >> (r & 31)) & 1u;
-      km |= (uint32_t)(!dead && p0 + q < n) << q;
-    }
+    uint32_t km = (~(sh.bits[r0 >> 5] >> (r0 & 31))) & ((1u << V) - 1u);
+    if (p0 + V > n) km &= p0 < n ? (1u << (n - p0)) - 1u : 0u;
     keep[j] = km;
     uint32_t tot;
     pre[j] = keep_count_prefix((uint32_t)__popc(km), &tot);
@@ -237,14 +282,32 @@ __global__ __launch_bounds__(WG) void rz_compact_vec(T* __restrict__ A, uint32_t
       if (idx < E) sh.off[idx] = ex;
       ex += loc[q];
     }
-    const uint32_t agg = __shfl(xs, kWave - 1, kWave);
+    if (lane == kWave - 1) sh.off[E] = xs;
+  }
+  __syncthreads();
+  // kept items in tile order into LDS (wave 0 stages its items, then looks back)
+#pragma unroll
+  for (int j = 0; j < VPT; ++j) stage_kept<T, V>(sh.stage, x[j], keep[j], sh.off[j * NW + wave] + pre[j]);
+  if (wave == 0) {
+    const uint32_t agg = sh.off[E];
     uint32_t excl = 0;
     if (t == 0) {
       if (lane == 0) st_relaxed(&status[0], pack_status(kFlagIncl, epoch, agg));
     } else {
       if (lane == 0) st_relaxed(&status[t], pack_status(kFlagAgg, epoch, agg));
+      // Wait on the nearest predecessor's word alone first (one load per poll,
+      // pause doubling to 16 x 64 cycles): polling whole windows from every
+      // waiting tile costs more fabric bandwidth than the moves.
+      uint32_t pause = 1, spins = 0;
+      while (true) {
+        const uint64_t st = ld_relaxed(&status[t - 1]);
+        if ((((uint32_t)(st >> 32) & 0x3fffffffu) == epoch && (st >> 62) != 0u) || ++spins > (1u << 22)) break;
+        for (uint32_t i = 0; i < pause; ++i) __builtin_amdgcn_s_sleep(1);
+        if (pause < 16u) pause <<= 1;
+      }
       int64_t jb = (int64_t)t - 1;
-      uint32_t spins = 0;
+      spins = 0;
+      pause = 1;
       while (true) {
         const int64_t q = jb - lane;
         uint64_t st = q >= 0 ? ld_relaxed(&status[q]) : pack_status(kFlagIncl, epoch, 0u);
@@ -258,7 +321,8 @@ __global__ __launch_bounds__(WG) void rz_compact_vec(T* __restrict__ A, uint32_t
             if (lane == 0) or_relaxed(&ctrl[kCtrlError], kErrLookbackTimeout);
             break;
           }
-          __builtin_amdgcn_s_sleep(1);
+          for (uint32_t i = 0; i < pause; ++i) __builtin_amdgcn_s_sleep(1);
+          if (pause < 16u) pause <<= 1;
           continue;
         }
         const uint32_t v = ((upto >> lane) & 1ull) ? (uint32_t)st : 0u;
@@ -271,16 +335,18 @@ __global__ __launch_bounds__(WG) void rz_compact_vec(T* __restrict__ A, uint32_t
     if (lane == 0) sh.excl = excl;
   }
   __syncthreads();
-  const uint32_t excl = sh.excl;
-#pragma unroll
-  for (int j = 0; j < VPT; ++j) {
-    uint32_t pos = excl + sh.off[j * NW + wave] + pre[j];
-#pragma unroll
-    for (int q = 0; q < V; ++q) {
-      if ((keep[j] >> q) & 1u) {
-        if (pos < n) A[pos] = x[j].e[q];
-        ++pos;
-      }
+  // A[excl, excl + agg) = stage[0, agg): whole aligned vectors, single items at the ends
+  const uint32_t lo = sh.excl, hi = lo + sh.off[E];
+  const uint32_t va = (lo + V - 1) / V * V, vb = hi / V * V;
+  if (va >= vb) {
+    for (uint32_t p = lo + tid; p < hi; p += WG) A[p] = sh.stage[p - lo];
+  } else {
+    for (uint32_t p = lo + tid; p < va; p += WG) A[p] = sh.stage[p - lo];
+    for (uint32_t p = vb + tid; p < hi; p += WG) A[p] = sh.stage[p - lo];
+    for (uint32_t p = va + (uint32_t)tid * V; p < vb; p += WG * V) {
+      uint4 v;
+      gather_vec<T, V>(sh.stage, p - lo, v);
+      *(uint4*)(A + p) = v;
     }
   }
 }
@@ -335,11 +401,20 @@ rz_status launch_validate(rz_ctx* ctx, const int32_t* R, int64_t n, int64_t k, u
 // launch
 // ---------------------------------------------------------------------------
 constexpr int kPartWG = 256, kPartIPT = 16;
+// Marks of lists of at least kPartBucketMinN items and kPartBucketMinK removals
+// are grouped by bitmap region first: there, one random atomic per removal
+// costs a DRAM read-modify-write (the bitmap is beyond the caches).  Measured
+// on 4-byte items: n=2^28, k=n/2 5.76 -> 1.64 ms; n=2^24, k=n/10
+// 131 -> 110 us; below 2^23 items the atomics stay in cache and win.
+constexpr int64_t kPartBucketMinK = 1 << 20, kPartBucketMinN = 1 << 23;
 
+// 512 threads x 8 vectors: 64 KiB tiles (staged in LDS); bigger tiles take
+// fewer look-back hops per byte.
+constexpr int kVecWG = 512, kVecVPT = 8;
 template <typename T>
 struct VecShape {
-  static constexpr int VPT = sizeof(T) == 4 ? 4 : 8;  // vectors per thread
-  static constexpr int TILE = kPartWG * VPT * (16 / (int)sizeof(T));
+  static constexpr int VPT = kVecVPT;  // vectors per thread
+  static constexpr int TILE = kVecWG * VPT * (16 / (int)sizeof(T));
 };
 
 template <typename T>
@@ -356,11 +431,18 @@ static rz_status partition_impl(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, 
   const int64_t gw = (k + 255) / 256;
   const dim3 g1((uint32_t)(gw < (int64_t)ctx->cu_count * 8 ? (gw > 0 ? gw : 1) : ctx->cu_count * 8));
   if ((st = prof_mark(ctx, flags, 0, s)) != RZ_OK) return st;
-  hipLaunchKernelGGL(rz_mark_all, g1, dim3(256), 0, s, R, (uint32_t)k, (uint32_t)n, bits, ctrl);
+  const bool bucketed = k >= kPartBucketMinK && n >= kPartBucketMinN && bucket_marks_fit(n);
+  if (bucketed) {
+    // every bitmap word is written; the epoch bump comes from an empty mark launch
+    if ((st = bucket_marks(ctx, R, k, 0u, n, bits, s)) != RZ_OK) return st;
+    hipLaunchKernelGGL(rz_mark_all, dim3(1), dim3(256), 0, s, R, 0u, (uint32_t)n, bits, ctrl);
+  } else {
+    hipLaunchKernelGGL(rz_mark_all, g1, dim3(256), 0, s, R, (uint32_t)k, (uint32_t)n, bits, ctrl);
+  }
   RZ_HIP(hipGetLastError());
   if ((st = prof_mark(ctx, flags, 1, s)) != RZ_OK) return st;
   if (vec)
-    hipLaunchKernelGGL((rz_compact_vec<T, kPartWG, VecShape<T>::VPT>), dim3((uint32_t)tiles), dim3(kPartWG), 0, s, A,
+    hipLaunchKernelGGL((rz_compact_vec<T, kVecWG, VecShape<T>::VPT>), dim3((uint32_t)tiles), dim3(kVecWG), 0, s, A,
                        (uint32_t)n, bits, (uint64_t*)ctx->status.ptr, ctrl);
   else
     hipLaunchKernelGGL((rz_compact<T, kPartWG, kPartIPT>), dim3((uint32_t)tiles), dim3(kPartWG), 0, s, A,

@@ -87,3 +87,15 @@ def test_partition_full_size(ctx, oracle):
     got = A.cpu().numpy()
     bad, fb = oracle.check(A_np, got, R_np, mode=1)
     assert bad == 0, (bad, fb)
+
+
+@pytest.mark.parametrize("esz", [4, 8, 16])
+def test_partition_bucketed_marks(ctx, oracle, esz):
+    """Lists of at least 2^23 items with at least 2^20 removals group their
+    marks by bitmap region (one LDS bitmap per 2^19 positions, every bitmap
+    word written whole); list lengths that end inside a region, clustered
+    removals that fill whole regions, and k = n - 1."""
+    for n, k, pattern in [((1 << 23) + 12345, 1 << 20, "uniform"), ((1 << 23) + 12345, 3 << 21, "clustered"),
+                          (1 << 23, (1 << 23) - 1, "uniform"), ((3 << 22) + 1, 1 << 22, "tail_heavy")]:
+        announce("partition bucketed", esz, n, k, pattern)
+        _run(ctx, oracle, W.payload(esz, n, seed=40), W.removals(pattern, n, k, seed=41))
