@@ -221,7 +221,10 @@ def bench_mixed(torch, ctx, args):
         return None
     disp = Dispatcher.load(ctx, MODEL_PATH)
     specs = MX.stream_specs(args.mixed_lists, seed=2)
-    res = MX.evaluate(ctx, disp, specs, 2 * 1000003, reps=3, log=log)
+    import redzone_amd as rz
+
+    untuned = rz.Context(ctx.device, load_tuned=False, load_dispatcher=False)
+    res = MX.evaluate(ctx, disp, specs, 2 * 1000003, reps=3, log=log, ctx_untuned=untuned)
     res.pop("_per_list_us")
     res.pop("_chosen")
     out = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res.items()}

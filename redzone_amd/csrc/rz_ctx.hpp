@@ -114,7 +114,7 @@ bool dense_bucket_marks(const rz_tile_config& cfg, int64_t k, uint32_t flags);
 // bucketed (LDS) marks of the window [base, base + m) into an m-bit bitmap (rz_dense.hip)
 bool bucket_marks_fit(int64_t m);
 rz_status bucket_marks(rz_ctx* ctx, const int32_t* R, int64_t k, uint32_t base, int64_t m, uint32_t* bits,
-                       hipStream_t s);
+                       bool zero_bits, hipStream_t s);
 // RZ_PROFILE: record mark i (0..3) on the stream when profiling
 rz_status prof_mark(rz_ctx* ctx, uint32_t flags, int i, hipStream_t s);
 // fill `count` uint32 words with `v` by a kernel (graph-capture safe, unlike hipMemsetAsync)

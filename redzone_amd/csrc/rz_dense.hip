@@ -51,7 +51,10 @@ namespace rz {
 constexpr uint32_t kDenseOrphan = 0x80000000u;
 constexpr int kCoarseMax = 1024;      // coarse destination bins and tail bins
 constexpr int kFineMax = 4096;        // fine destination bins
-constexpr uint32_t kTailBinLog = 19;  // tail positions per tail bin (a 64 KiB LDS bitmap)
+constexpr uint32_t kTailBinLog = 19;    // tail positions per tail bin of the dense form (a 64 KiB LDS bitmap)
+constexpr uint32_t kBucketBinLog = 19;  // positions per bin of bucket_marks (a 64 KiB LDS bitmap)
+// rz_dense_tail_mark's dynamic-LDS limit, set once for both of its users
+constexpr size_t kTailMarkLdsMax = (size_t)4 << (kBucketBinLog - 5);
 constexpr uint32_t kCoarseTarget = 64;
 
 __device__ __forceinline__ uint32_t classify(uint32_t d, bool dead, uint32_t sZ) {
@@ -443,7 +446,7 @@ __global__ __launch_bounds__(1024) void rz_dense_scatter(T* __restrict__ A, cons
 template <int WG, int IPT>
 // k removals; marked positions [sZ, sZ + m) (m = k for the red zone)
 __global__ __launch_bounds__(WG) void rz_dense_tail_count(const int32_t* __restrict__ R, uint32_t k, uint32_t sZ,
-                                                           uint32_t m, uint32_t L, uint32_t NBt,
+                                                           uint32_t m, uint32_t L, uint32_t NBt, uint32_t blog,
                                                            uint32_t* __restrict__ cnt) {
   __shared__ uint32_t hist[kCoarseMax];
   const uint32_t tid = threadIdx.x, j = blockIdx.x;
@@ -458,8 +461,7 @@ __global__ __launch_bounds__(WG) void rz_dense_tail_count(const int32_t* __restr
       z[q] = p < b ? (uint32_t)R[p] - sZ : 0xffffffffu;  // wraps to >= m below the window
     }
 #pragma unroll
-    for (int q = 0; q < IPT; ++q)
-      if (z[q] < m) atomicAdd(&hist[z[q] >> kTailBinLog], 1u);
+    for (int q = 0; q < IPT; ++q) lds_bin_count(hist, z[q] >> blog, z[q] < m);
   }
   __syncthreads();
   for (uint32_t q = tid; q < NBt; q += WG) cnt[(size_t)j * NBt + q] = hist[q];
@@ -468,7 +470,7 @@ __global__ __launch_bounds__(WG) void rz_dense_tail_count(const int32_t* __restr
 // Dynamic LDS: S red-zone offsets and S bin ids (S = WG * IPT).
 template <int WG, int IPT>
 __global__ __launch_bounds__(WG) void rz_dense_tail_entries(const int32_t* __restrict__ R, uint32_t k, uint32_t sZ,
-                                                             uint32_t m, uint32_t L, uint32_t NBt,
+                                                             uint32_t m, uint32_t L, uint32_t NBt, uint32_t blog,
                                                              const uint32_t* __restrict__ off,
                                                              const uint32_t* __restrict__ binbase,
                                                              uint32_t* __restrict__ ent) {
@@ -493,7 +495,7 @@ __global__ __launch_bounds__(WG) void rz_dense_tail_entries(const int32_t* __res
       z[q] = p < b ? (uint32_t)R[p] - sZ : 0xffffffffu;
     }
 #pragma unroll
-    for (int q = 0; q < IPT; ++q) rk[q] = z[q] < m ? atomicAdd(&lcnt[z[q] >> kTailBinLog], 1u) : 0u;
+    for (int q = 0; q < IPT; ++q) rk[q] = lds_bin_add(lcnt, z[q] >> blog, z[q] < m);
     __syncthreads();
     for (uint32_t q = tid; q < NBt; q += WG) {
       gb[q] = gcur[q];
@@ -504,7 +506,7 @@ __global__ __launch_bounds__(WG) void rz_dense_tail_entries(const int32_t* __res
 #pragma unroll
     for (int q = 0; q < IPT; ++q) {
       if (z[q] >= m) continue;
-      const uint32_t bin = z[q] >> kTailBinLog;
+      const uint32_t bin = z[q] >> blog;
       const uint32_t e = lcnt[bin] + rk[q];
       sz[e] = z[q];
       sb[e] = (uint16_t)bin;
@@ -520,32 +522,43 @@ __global__ __launch_bounds__(WG) void rz_dense_tail_entries(const int32_t* __res
   }
 }
 
-// One workgroup per tail bin: set the bin's bits in a dynamic-LDS bitmap of
-// 2^kTailBinLog bits, then write its words whole (every word of the m-bit
-// bitmap belongs to exactly one bin).
+// G workgroups per tail bin (workgroup g of bin b is block b*G + g): each
+// sets the bits of its share of the bin's entries in a dynamic-LDS bitmap of
+// 2^blog bits.  With G = 1 it writes the bin's words whole (every word
+// of the m-bit bitmap belongs to exactly one bin, so the bitmap need not be
+// clear beforehand); with G > 1 it ORs its nonzero words in (the bitmap must
+// be zero beforehand).
 __global__ __launch_bounds__(1024) void rz_dense_tail_mark(const uint32_t* __restrict__ ent,
                                                           const uint32_t* __restrict__ binbase, uint32_t m,
                                                           uint32_t* __restrict__ bits,
-                                                          uint64_t* __restrict__ stat_tail, uint32_t NBt) {
-  constexpr uint32_t W = 1u << (kTailBinLog - 5);
+                                                          uint64_t* __restrict__ stat_tail, uint32_t NBt,
+                                                          uint32_t G, uint32_t blog) {
+  const uint32_t W = 1u << (blog - 5);
   extern __shared__ __align__(16) unsigned char dyn[];
   uint32_t* lb = (uint32_t*)dyn;
-  const uint32_t bin = blockIdx.x, tid = threadIdx.x;
+  const uint32_t bin = blockIdx.x / G, g = blockIdx.x % G, tid = threadIdx.x;
   for (uint32_t w = tid; w < W; w += 1024u) lb[w] = 0u;
   __syncthreads();
-  const uint32_t z0 = bin << kTailBinLog;
-  const uint32_t s = binbase[bin], e = binbase[bin + 1];
+  const uint32_t z0 = bin << blog;
+  const uint32_t s0 = binbase[bin], e0 = binbase[bin + 1];
+  const uint32_t share = (e0 - s0 + G - 1) / G;
+  const uint32_t s = s0 + min(g * share, e0 - s0), e = s0 + min((g + 1) * share, e0 - s0);
   for (uint32_t q = s + tid; q < e; q += 1024u) {
-    const uint32_t r = ent[q] - z0;  // < 2^kTailBinLog for an entry of this bin
-    if (r < (1u << kTailBinLog)) atomicOr(&lb[r >> 5], 1u << (r & 31));
+    const uint32_t r = ent[q] - z0;  // < 2^blog for an entry of this bin
+    if (r < (1u << blog)) atomicOr(&lb[r >> 5], 1u << (r & 31));
   }
   __syncthreads();
   const uint32_t nw = (uint32_t)(((uint64_t)m + 31u) >> 5);
   for (uint32_t w = tid; w < W; w += 1024u) {
     const uint32_t gw = (z0 >> 5) + w;
-    if (gw < nw) bits[gw] = lb[w];
+    if (gw < nw) {
+      if (G == 1)
+        bits[gw] = lb[w];
+      else if (lb[w])
+        or_relaxed(&bits[gw], lb[w]);
+    }
   }
-  if (stat_tail && bin == 0 && tid == 0) *stat_tail = binbase[NBt];
+  if (stat_tail && blockIdx.x == 0 && tid == 0) *stat_tail = binbase[NBt];
 }
 
 // ---------------------------------------------------------------------------
@@ -587,7 +600,7 @@ static rz_status dense_launch(rz_ctx* ctx, T* A, const int32_t* R, uint32_t k, u
     RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_tail_entries<WG, IPT>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_tail));
     RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_tail_mark, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds_mark));
+                               (int)kTailMarkLdsMax));
     attr_set = true;
   }
   uint32_t* cnt = (uint32_t*)ctx->dense_cnt.ptr;
@@ -607,14 +620,15 @@ static rz_status dense_launch(rz_ctx* ctx, T* A, const int32_t* R, uint32_t k, u
   if (d.NBt) {
     // the red-zone offsets use ent_d's space before the head writes do
     uint32_t* toff = cnt + (size_t)d.nt * d.NBt;
-    hipLaunchKernelGGL((rz_dense_tail_count<WG, IPT>), dim3(d.nt), dim3(WG), 0, s, R, k, sZ, k, d.L, d.NBt, cnt);
+    hipLaunchKernelGGL((rz_dense_tail_count<WG, IPT>), dim3(d.nt), dim3(WG), 0, s, R, k, sZ, k, d.L, d.NBt,
+                       kTailBinLog, cnt);
     hipLaunchKernelGGL(rz_dense_colscan, dim3(d.NBt / 32), dim3(1024), 0, s, cnt, d.nt, d.NBt, toff, tbtot);
     hipLaunchKernelGGL(rz_dense_topscan, dim3(1), dim3(1024), 0, s, tbtot, d.NBt, tbinbase, ocnt, 0u,
                        (uint64_t*)nullptr, ctrl, 0);
     hipLaunchKernelGGL((rz_dense_tail_entries<WG, IPT>), dim3(d.nt), dim3(WG), lds_tail, s, R, k, sZ, k, d.L,
-                       d.NBt, toff, tbinbase, ent_d);
+                       d.NBt, kTailBinLog, toff, tbinbase, ent_d);
     hipLaunchKernelGGL(rz_dense_tail_mark, dim3(d.NBt), dim3(1024), lds_mark, s, ent_d, tbinbase, k, bits,
-                       stats ? (uint64_t*)(ctrl + kCtrlStatTail) : (uint64_t*)nullptr, d.NBt);
+                       stats ? (uint64_t*)(ctrl + kCtrlStatTail) : (uint64_t*)nullptr, d.NBt, 1u, kTailBinLog);
   }
   uint32_t* off = cnt + (size_t)d.nt * d.NB;
   hipLaunchKernelGGL((rz_dense_count<WG, IPT>), dim3(d.nt), dim3(WG), 0, s, R, k, sZ, d.L, bits, d.NB, d.bs, cnt,
@@ -633,35 +647,40 @@ static rz_status dense_launch(rz_ctx* ctx, T* A, const int32_t* R, uint32_t k, u
 }
 
 // Bucketed marks of the window [base, base + m): the k removals are grouped by
-// bin of 2^kTailBinLog positions (count, scan, staged entries), then one
+// bin of 2^kBucketBinLog positions (count, scan, staged entries), then one
 // workgroup per bin sets its bits in LDS and writes its words whole.  Every
 // word of the m-bit bitmap is written (zero where nothing is removed), so
-// the bitmap need not be clear beforehand.  Removals outside the window are
-// ignored.  Needs m <= kCoarseMax * 2^kTailBinLog.
-bool bucket_marks_fit(int64_t m) { return m > 0 && m <= ((int64_t)kCoarseMax << kTailBinLog); }
+// the bitmap need not be clear beforehand.  With zero_bits (the bitmap is
+// zero on entry) a bin's marks are shared by several workgroups that OR
+// their words in, so that about two workgroups per CU run.  Removals outside
+// the window are ignored.  Needs m <= kCoarseMax * 2^kBucketBinLog.
+bool bucket_marks_fit(int64_t m) { return m > 0 && m <= ((int64_t)kCoarseMax << kBucketBinLog); }
 
 rz_status bucket_marks(rz_ctx* ctx, const int32_t* R, int64_t k, uint32_t base, int64_t m, uint32_t* bits,
-                       hipStream_t s) {
+                       bool zero_bits, hipStream_t s) {
+  // (2^20-position bins and 16 removals per thread per step write longer
+  // runs, but their LDS halves the workgroups per CU: 30-60% slower)
   constexpr int WG = 1024, IPT = 8;
   constexpr uint32_t S = WG * IPT;
   if (!bucket_marks_fit(m) || k <= 0) return RZ_EINVAL;
   rz_tile_config cfg{WG, IPT, 1024, 5, 0, 2};
   uint32_t L, nt;
   slice_shape(k, cfg, kMaxSlices, &L, &nt);
-  uint32_t NBt = (uint32_t)((m + ((int64_t)1 << kTailBinLog) - 1) >> kTailBinLog);
+  uint32_t NBt = (uint32_t)((m + ((int64_t)1 << kBucketBinLog) - 1) >> kBucketBinLog);
   NBt = (NBt + 31u) & ~31u;
   rz_status st;
   if ((st = ensure(ctx, ctx->dense_cnt, (size_t)nt * NBt * 4 * 2, s)) != RZ_OK) return st;
   if ((st = ensure(ctx, ctx->dense_small, kSmallBytes, s)) != RZ_OK) return st;
   if ((st = ensure(ctx, ctx->dense_ent, (size_t)k * 4, s)) != RZ_OK) return st;
   const size_t lds_tail = (size_t)S * (4 + 2);
-  const size_t lds_mark = (size_t)4 << (kTailBinLog - 5);
+  const size_t lds_mark = (size_t)4 << (kBucketBinLog - 5);
+  if ((size_t)ctx->lds_max < lds_mark || (size_t)ctx->lds_max < lds_tail + 16 * 1024) return RZ_EINVAL;
   static bool attr_set = false;
   if (!attr_set) {
     RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_tail_entries<WG, IPT>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_tail));
     RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_tail_mark, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds_mark));
+                               (int)kTailMarkLdsMax));
     attr_set = true;
   }
   uint32_t* cnt = (uint32_t*)ctx->dense_cnt.ptr;
@@ -670,14 +689,21 @@ rz_status bucket_marks(rz_ctx* ctx, const int32_t* R, int64_t k, uint32_t base, 
   uint32_t* tbinbase = tbtot + kSmallStride;
   uint32_t* ent = (uint32_t*)ctx->dense_ent.ptr;
   hipLaunchKernelGGL((rz_dense_tail_count<WG, IPT>), dim3(nt), dim3(WG), 0, s, R, (uint32_t)k, base, (uint32_t)m, L,
-                     NBt, cnt);
+                     NBt, kBucketBinLog, cnt);
   hipLaunchKernelGGL(rz_dense_colscan, dim3(NBt / 32), dim3(1024), 0, s, cnt, nt, NBt, toff, tbtot);
   hipLaunchKernelGGL(rz_dense_topscan, dim3(1), dim3(1024), 0, s, tbtot, NBt, tbinbase, (const uint64_t*)nullptr, 0u,
                      (uint64_t*)nullptr, (uint32_t*)nullptr, 0);
   hipLaunchKernelGGL((rz_dense_tail_entries<WG, IPT>), dim3(nt), dim3(WG), lds_tail, s, R, (uint32_t)k, base,
-                     (uint32_t)m, L, NBt, toff, tbinbase, ent);
-  hipLaunchKernelGGL(rz_dense_tail_mark, dim3(NBt), dim3(1024), lds_mark, s, ent, tbinbase, (uint32_t)m, bits,
-                     (uint64_t*)nullptr, NBt);
+                     (uint32_t)m, L, NBt, kBucketBinLog, toff, tbinbase, ent);
+  uint32_t G = 1;
+  if (zero_bits) {
+    const uint32_t want = 2u * (uint32_t)ctx->cu_count;
+    G = (want + NBt - 1) / NBt;
+    if (G > 16) G = 16;
+    if (G < 1) G = 1;
+  }
+  hipLaunchKernelGGL(rz_dense_tail_mark, dim3(NBt * G), dim3(1024), lds_mark, s, ent, tbinbase, (uint32_t)m, bits,
+                     (uint64_t*)nullptr, NBt, G, kBucketBinLog);
   RZ_HIP(hipGetLastError());
   return RZ_OK;
 }

@@ -29,6 +29,36 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
+// LDS counter increment with a rank: every active lane adds one to cnt[bin]
+// and gets a rank that is unique within the bin.  When all active lanes of
+// the wave share one bin (a run of removals), one atomic adds them all; else
+// each lane takes its own (grouping partly shared bins costs more on uniform
+// lists than it saves on clustered ones).  Call from every lane of the wave.
+__device__ __forceinline__ uint32_t lds_bin_add(uint32_t* cnt, uint32_t bin, bool active) {
+  const uint64_t act = ballot(active);
+  if (!act) return 0u;
+  const int leader = __builtin_ctzll(act);
+  const uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)bin, leader);
+  if (ballot(active && bin == lb) == act) {
+    uint32_t base = 0u;
+    if (lane_id() == leader) base = atomicAdd(&cnt[lb], (uint32_t)__popcll(act));
+    return (uint32_t)__builtin_amdgcn_readlane((int)base, leader) + lane_rank(act);
+  }
+  return active ? atomicAdd(&cnt[bin], 1u) : 0u;
+}
+// Count only (no ranks), grouped the same way.
+__device__ __forceinline__ void lds_bin_count(uint32_t* cnt, uint32_t bin, bool active) {
+  const uint64_t act = ballot(active);
+  if (!act) return;
+  const int leader = __builtin_ctzll(act);
+  const uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)bin, leader);
+  if (ballot(active && bin == lb) == act) {
+    if (lane_id() == leader) atomicAdd(&cnt[lb], (uint32_t)__popcll(act));
+  } else if (active) {
+    atomicAdd(&cnt[bin], 1u);
+  }
+}
+
 __device__ __forceinline__ uint32_t ld_relaxed(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

@@ -434,7 +434,8 @@ static rz_status partition_impl(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, 
   const bool bucketed = k >= kPartBucketMinK && n >= kPartBucketMinN && bucket_marks_fit(n);
   if (bucketed) {
     // every bitmap word is written; the epoch bump comes from an empty mark launch
-    if ((st = bucket_marks(ctx, R, k, 0u, n, bits, s)) != RZ_OK) return st;
+    // the bitmap is zero here: every call leaves the words it used clear
+    if ((st = bucket_marks(ctx, R, k, 0u, n, bits, true, s)) != RZ_OK) return st;
     hipLaunchKernelGGL(rz_mark_all, dim3(1), dim3(256), 0, s, R, 0u, (uint32_t)n, bits, ctrl);
   } else {
     hipLaunchKernelGGL(rz_mark_all, g1, dim3(256), 0, s, R, (uint32_t)k, (uint32_t)n, bits, ctrl);

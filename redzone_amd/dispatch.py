@@ -108,7 +108,8 @@ class Dispatcher:
         out = torch.empty((cnt, NF), dtype=torch.float32, device=f"cuda:{self.ctx.device}")
         self.ctx._call("rz_features", self.ctx.handle, _ptr(d_descs), cnt, _ptr(out),
                        _stream_ptr(stream, self.ctx.device))
-        torch.cuda.current_stream(self.ctx.device).synchronize()
+        if stream is not None:
+            d_descs.record_stream(stream)  # keep the upload alive until the kernel has read it
         return out
 
     # ---- inference ------------------------------------------------------

@@ -196,43 +196,72 @@ def run_plain(ctx, lists, specs=None):
 # whatever method the network prefers for them on their own: a list's own
 # launch costs about 10 us, a list inside a batch about half a microsecond, and
 # the queue moves large lists at the same random-write rate as their own
-# launch.  Larger lists are launched on their own with the chosen method.
-BATCH_MAX_K = 1 << 24
+# launch.  Larger lists are launched on their own with the chosen method
+# (measured on the 10,000-list stream: 2^20 175 ms, 2^22 170 ms, 2^24 190 ms).
+BATCH_MAX_K = 1 << 22
+
+
+def _side_stream(ctx):
+    import torch
+
+    st = getattr(ctx, "_side_stream", None)
+    if st is None:
+        st = ctx._side_stream = torch.cuda.Stream(ctx.device)  # default priority
+    return st
 
 
 def run_batched_learned(ctx, disp, lists, specs=None, batch_max_k=BATCH_MAX_K):
-    """Features + MLP choice on the device; lists below batch_max_k removals in
-    one persistent launch, the others with their chosen method; wall time (s)
-    of everything including the decision, the descriptor upload and the batch
-    plan.  lists: [(data, removals), ...] or a ListDescs."""
+    """Lists below batch_max_k removals in one persistent launch (Redzone);
+    the others launched on their own with the method the MLP picks.  The
+    features and the forward pass of the large lists run on a second stream
+    while the batch runs, and only the choice comes back to the host.  Wall
+    time (s) of everything including the decision, the descriptor upload and
+    the batch plan; returns (wall, method that ran per list).
+    lists: [(data, removals), ...] or a ListDescs."""
     import torch
 
+    from .dispatch import rule_choice
+
     descs = _as_descs(ctx, lists)
-    k = descs.rows["k"]
+    n, k = descs.rows["n"], descs.rows["k"]
     small = np.nonzero(k < batch_max_k)[0]
     large = np.nonzero(k >= batch_max_k)[0]
+    choice = np.zeros(len(descs), np.int32)  # the batch runs Redzone
     s = _stream_arg(ctx)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    X = disp.features(descs)
-    choice = disp.choose(X, descs.rows["n"], k)
+    side = _side_stream(ctx)
+    C = None
+    if len(large) and disp.trained:
+        with torch.cuda.stream(side):
+            _, C = disp.predict(disp.features(descs.subset(large), stream=side), stream=side)
     if len(small):
         ctx.remove_batched(descs.subset(small))
     if len(large):
+        if C is not None:
+            with torch.cuda.stream(side):
+                choice[large] = C.cpu().numpy()
+        else:
+            choice[large] = [rule_choice(a, b) for a, b in zip(n[large], k[large])]
         _launch_each(ctx, descs.rows[large].tolist(), [int(choice[i]) for i in large], s)
     torch.cuda.synchronize()
     return time.perf_counter() - t0, choice
 
 
-def evaluate(ctx, disp, specs, seed, reps=3, budget=8 << 30, log=None, sweep=(1 << 20, 1 << 22)):
+def evaluate(ctx, disp, specs, seed, reps=3, budget=8 << 30, log=None, sweep=(1 << 20, 1 << 24), ctx_untuned=None):
     """Mixed-stream evaluation; returns a dict of totals and regrets.  `sweep`
-    lists further batch thresholds whose batched wall time is also measured."""
+    lists further batch thresholds whose batched wall time is also measured.
+    With ctx_untuned (a Context without the tuned table: every list runs the
+    fixed default shape, three kernels) the plain per-list launches are also
+    timed on it."""
     import torch
 
     from .dispatch import METHODS, rule_choice
 
     tot = {"plain_rule_wall_s": 0.0, "batched_learned_wall_s": 0.0, "oracle_best_us": 0.0, "learned_us": 0.0,
            "rule_us": 0.0, "redzone_only_us": 0.0}
+    if ctx_untuned is not None:
+        tot["plain_untuned_wall_s"] = 0.0
     swept = {int(b): 0.0 for b in sweep}
     picks = np.zeros(4, np.int64)
     per_list = np.zeros((len(specs), 4), np.float64)
@@ -262,14 +291,21 @@ def evaluate(ctx, disp, specs, seed, reps=3, budget=8 << 30, log=None, sweep=(1 
                 r.copy_(r0)
             torch.cuda.synchronize()
 
-        restore()
-        tot["plain_rule_wall_s"] += run_plain(ctx, wdescs)
-        restore()
-        wall, _ = run_batched_learned(ctx, disp, wdescs)
-        tot["batched_learned_wall_s"] += wall
-        for b in swept:
+        # every path runs once untimed first, so that its scratch buffers have
+        # grown to this group's sizes before the timed run
+        def timed(fn):
             restore()
-            swept[b] += run_batched_learned(ctx, disp, wdescs, batch_max_k=b)[0]
+            fn()
+            restore()
+            return fn()
+
+        tot["plain_rule_wall_s"] += timed(lambda: run_plain(ctx, wdescs))
+        if ctx_untuned is not None:
+            udescs = ctx_untuned.descs(work)
+            tot["plain_untuned_wall_s"] += timed(lambda: run_plain(ctx_untuned, udescs))
+        tot["batched_learned_wall_s"] += timed(lambda: run_batched_learned(ctx, disp, wdescs)[0])
+        for b in swept:
+            swept[b] += timed(lambda: run_batched_learned(ctx, disp, wdescs, batch_max_k=b)[0])
         del pristine, work
         torch.cuda.empty_cache()
         if log:
@@ -277,6 +313,8 @@ def evaluate(ctx, disp, specs, seed, reps=3, budget=8 << 30, log=None, sweep=(1 
     tot["learned_regret"] = tot["learned_us"] / tot["oracle_best_us"] - 1
     tot["rule_regret"] = tot["rule_us"] / tot["oracle_best_us"] - 1
     tot["batched_learned_vs_plain"] = tot["plain_rule_wall_s"] / tot["batched_learned_wall_s"]
+    if ctx_untuned is not None:
+        tot["batched_learned_vs_plain_untuned"] = tot["plain_untuned_wall_s"] / tot["batched_learned_wall_s"]
     tot["batch_max_k"] = BATCH_MAX_K
     tot["batched_wall_s_by_batch_max_k"] = {str(b): w for b, w in swept.items()}
     tot["picks"] = {m: int(c) for m, c in zip(METHODS, picks)}
@@ -321,7 +359,8 @@ def main():
         count, seed = args.lists or 10000, 2 if args.seed is None else args.seed
         disp = Dispatcher.load(ctx, args.model)
         specs = stream_specs(count, seed)
-        res = evaluate(ctx, disp, specs, seed * 1000003, args.reps, log=log)
+        res = evaluate(ctx, disp, specs, seed * 1000003, args.reps, log=log,
+                       ctx_untuned=rz.Context(ctx.device, load_tuned=False, load_dispatcher=False))
         per_list, chosen = res.pop("_per_list_us"), res.pop("_chosen")
         if args.details:
             np.savez(args.details, specs=np.array(specs, dtype=object).astype(str), times_us=per_list, chosen=chosen)
