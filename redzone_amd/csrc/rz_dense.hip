@@ -128,7 +128,7 @@ __global__ __launch_bounds__(WG) void rz_dense_count(const int32_t* __restrict__
     for (int q = 0; q < IPT; ++q) {
       const uint32_t p = base + q * WG + tid;
       const uint32_t cls = classify(d[q], (wv[q] >> (p & 31)) & 1u, sZ);
-      if (cls == kMove || cls == kDstOrphan) atomicAdd(&hist[d[q] >> bs], 1u);
+      lds_bin_count(hist, d[q] >> bs, cls == kMove || cls == kDstOrphan);
       nd += cls == kDstOrphan;
       ns += cls == kSrcOrphan;
     }
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(WG) void rz_dense_entries(const T* __restrict__ A, 
       const uint64_t md = ballot(cls == kDstOrphan);
       info[q] = cls | (lane_rank(ms) << 8) | (lane_rank(md) << 16);
       if (lane == 0) s_off[q * NW + wave] = (uint32_t)__popcll(ms) | ((uint32_t)__popcll(md) << 16);
-      rk[q] = (cls == kMove || cls == kDstOrphan) ? atomicAdd(&lcnt[d[q] >> bs], 1u) : 0u;
+      rk[q] = lds_bin_add(lcnt, d[q] >> bs, cls == kMove || cls == kDstOrphan);
     }
     __syncthreads();
     if (wave == 0) {
@@ -378,7 +378,9 @@ __global__ __launch_bounds__(WG) void rz_dense_refine(const uint32_t* __restrict
 #pragma unroll
   for (int q = 0; q < IPT; ++q) {
     const uint32_t f = (d[q] & ~kDenseOrphan) >> bsf;
-    rk[q] = (d[q] != 0xffffffffu && f < NBf) ? atomicAdd(&lcnt[f], 1u) : 0xffffffffu;
+    const bool valid = d[q] != 0xffffffffu && f < NBf;
+    rk[q] = lds_bin_add(lcnt, f, valid);
+    if (!valid) rk[q] = 0xffffffffu;
   }
   __syncthreads();
   for (uint32_t q = tid; q < NBf; q += WG) {

@@ -4,9 +4,10 @@ Context loads by default (redzone_amd/tuned_gfx950.txt).
 
     python -m redzone_amd.tune [--out PATH] [--reps 5] [--quick]
 
-For every element size (4, 8, 16 bytes) and every k bucket 2^b (k < n/2) the
-native tuner (rz_tune) times the built-in candidate shapes on a synthetic
-list of 1 GiB with cold caches and keeps the fastest; then the batch tuner
+For every element size (4, 8, 16 bytes) and every k bucket [2^b, 2^(b+1))
+(up to k = n/2) the native tuner (rz_tune) times the built-in candidate
+shapes at k = 2^(b+1/2) on a synthetic list of 1 GiB with cold caches and
+keeps the fastest; then the batch tuner
 (rz_tune_batch) picks the queue shape on a 4096-list batch shaped like the
 batched benchmark.  Every line of the result is printed as JSON.
 """
@@ -35,7 +36,7 @@ def main():
         n = (1 << 30) // esz
         hi_b = lo_b
         for b in range(lo_b, 32):
-            k = 1 << b
+            k = int(2 ** (b + 0.5))  # the middle of bucket [2^b, 2^(b+1)) on a log scale
             if k * 2 > n:
                 break
             best, ms, results = ctx.tune(esz, n, k, reps=args.reps, cold=True)
