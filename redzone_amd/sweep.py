@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Timing sweeps of the Redzone path on one device, one JSON line per point.
 
-    python -m redzone_amd.sweep [--config 2|3|all] [--reps 5]
+    python -m redzone_amd.sweep [--config 2|3|gather|partition|all] [--reps 5]
 
 Config 2: n = 2^28 unique uint32 payloads, k/n in {1e-4, 1e-3, 1e-2, 0.1, 0.5},
 removals uniform / clustered (geometric runs, mean 256) / tail-heavy (Beta(5,1)
@@ -16,6 +16,9 @@ Gather: n = 2^28 uint32, a sorted uniform preserve list of p = n * keep
 positions (keep in {0.001, 0.01, 0.1, 0.5, 0.99}) timed with the in-place
 gather, and its complement as a removal list timed with Redzone and the O(n)
 partition.
+
+Partition: the O(n) stable partition on n = 2^28 uint32 at k/n = 0.3% and
+50% uniform (the two mark forms), beside Redzone on the same list.
 
 Every call starts from pristine copies of the list and the removal list, with
 a 1 GiB read in between so that the caches hold no useful lines (cold
@@ -144,9 +147,34 @@ def sweep_gather(ctx, rz, W, torch, reps, emit):
         torch.cuda.empty_cache()
 
 
+def sweep_partition(ctx, rz, W, torch, reps, emit):
+    """The O(n) stable partition alone on n = 2^28 uint32 with uniform removals
+    at k/n = 0.3% (atomic marks) and 50% (bucketed marks), beside Redzone."""
+    n = 1 << 28
+    time_call = _timer(torch)
+    A0 = W.payload(4, n, seed=2026, device="cuda")
+    A = torch.empty_like(A0)
+    for frac in (0.003, 0.5):
+        k = int(n * frac)
+        R0 = W.uniform_removals(n, k, seed=2031, device="cuda")
+        R = torch.empty_like(R0)
+
+        def restore():
+            A.copy_(A0)
+            R.copy_(R0)
+
+        t_on = time_call(restore, lambda: ctx.partition(A, R), reps)
+        t_rz = time_call(restore, lambda: ctx.remove(A, R), reps)
+        emit({"config": "partition", "n": n, "frac": frac, "k": k, "partition_ms": round(t_on, 4),
+              "redzone_ms": round(t_rz, 4),
+              "partition_GBps": round((4 * n + 4 * (n - k)) / (t_on * 1e-3) / 1e9, 1)})
+        del R0, R
+        torch.cuda.empty_cache()
+
+
 def main():
     p = argparse.ArgumentParser()
-    p.add_argument("--config", default="all", choices=["2", "3", "gather", "all"])
+    p.add_argument("--config", default="all", choices=["2", "3", "gather", "partition", "all"])
     p.add_argument("--reps", type=int, default=5)
     args = p.parse_args()
     import torch
@@ -165,6 +193,8 @@ def main():
         sweep_config3(ctx, rz, W, torch, args.reps, emit)
     if args.config in ("gather", "all"):
         sweep_gather(ctx, rz, W, torch, args.reps, emit)
+    if args.config in ("partition", "all"):
+        sweep_partition(ctx, rz, W, torch, args.reps, emit)
 
 
 if __name__ == "__main__":
