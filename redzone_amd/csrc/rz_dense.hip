@@ -545,9 +545,18 @@ __global__ __launch_bounds__(1024) void rz_dense_tail_mark(const uint32_t* __res
   const uint32_t s0 = binbase[bin], e0 = binbase[bin + 1];
   const uint32_t share = (e0 - s0 + G - 1) / G;
   const uint32_t s = s0 + min(g * share, e0 - s0), e = s0 + min((g + 1) * share, e0 - s0);
-  for (uint32_t q = s + tid; q < e; q += 1024u) {
-    const uint32_t r = ent[q] - z0;  // < 2^blog for an entry of this bin
-    if (r < (1u << blog)) atomicOr(&lb[r >> 5], 1u << (r & 31));
+  // 8 loads in flight per thread, then their LDS atomics
+  constexpr int U = 8;
+  for (uint32_t q0 = s + tid; q0 < e; q0 += 1024u * U) {
+    uint32_t r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t q = q0 + u * 1024u;
+      r[u] = q < e ? ent[q] - z0 : 0xffffffffu;  // < 2^blog for an entry of this bin
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (r[u] < (1u << blog)) atomicOr(&lb[r[u] >> 5], 1u << (r[u] & 31));
   }
   __syncthreads();
   const uint32_t nw = (uint32_t)(((uint64_t)m + 31u) >> 5);
