@@ -23,20 +23,21 @@
 //     colscan / topscan   coarse bin bases, per-tile offsets, per-tile orphan
 //               ranks (exclusive, in i order)
 //     entries   tile j again, in steps staged through LDS: (position, value)
-//               runs per coarse bin; a dest orphan carries its rank t instead
-//               of a value (top bit of the position set), and the t-th source
-//               orphan's value goes to src[t].  Orphan ranks follow i, so the
-//               pairing is the canonical one (redzone.hxx:834-934).
+//               runs of the moves per coarse bin; the t-th dest orphan's
+//               position goes to dpos[t] and the t-th source orphan's value
+//               to src[t].  Orphan ranks follow i, so the pairing is the
+//               canonical one (redzone.hxx:834-934).
 //     refine    chunks of the coarse runs, staged through LDS by fine bin
 //               (about 1 MiB of the list each); runs go to fixed-capacity fine
 //               bins through one atomic cursor add per (chunk, fine bin).
 //               Distinct removals write each position at most once, so a fine
 //               bin never holds more entries than it has positions.
+//     refine    again, on the orphan pairs (dpos[t], src[t]) read in rank
+//               order, into the same fine bins
 //     scatter   each fine bin is split among G workgroups laid out to share
 //               one XCD (workgroups are dispatched to the 8 XCDs round robin),
 //               bins taken in order, so each XCD writes about two bins at a
-//               time and the writes combine in its L2: A[d] = value, or src[t]
-//               for a dest orphan.
+//               time and the writes combine in its L2: A[d] = value.
 //
 // Counts and offsets are recomputed on every call.  The fine cursors are
 // zeroed by the count kernel of the next call; the tail bitmap is overwritten
@@ -48,7 +49,6 @@
 
 namespace rz {
 
-constexpr uint32_t kDenseOrphan = 0x80000000u;
 constexpr int kCoarseMax = 1024;      // coarse destination bins and tail bins
 constexpr int kFineMax = 4096;        // fine destination bins
 constexpr uint32_t kTailBinLog = 19;    // tail positions per tail bin of the dense form (a 64 KiB LDS bitmap)
@@ -89,14 +89,6 @@ __device__ uint32_t lds_excl_scan(uint32_t* a, uint32_t N, uint32_t* tmp) {
   return total;
 }
 
-template <typename T> __device__ __forceinline__ T rank_value(uint32_t t);
-template <> __device__ __forceinline__ uint32_t rank_value<uint32_t>(uint32_t t) { return t; }
-template <> __device__ __forceinline__ uint64_t rank_value<uint64_t>(uint32_t t) { return t; }
-template <> __device__ __forceinline__ uint4 rank_value<uint4>(uint32_t t) { return make_uint4(t, 0u, 0u, 0u); }
-template <typename T> __device__ __forceinline__ uint32_t value_rank(const T& v);
-template <> __device__ __forceinline__ uint32_t value_rank<uint32_t>(const uint32_t& v) { return v; }
-template <> __device__ __forceinline__ uint32_t value_rank<uint64_t>(const uint64_t& v) { return (uint32_t)v; }
-template <> __device__ __forceinline__ uint32_t value_rank<uint4>(const uint4& v) { return v.x; }
 
 // ---- count ------------------------------------------------------------------
 // Also zeroes the fine-bin cursors that this call's refine kernel adds to.
@@ -128,7 +120,7 @@ __global__ __launch_bounds__(WG) void rz_dense_count(const int32_t* __restrict__
     for (int q = 0; q < IPT; ++q) {
       const uint32_t p = base + q * WG + tid;
       const uint32_t cls = classify(d[q], (wv[q] >> (p & 31)) & 1u, sZ);
-      lds_bin_count(hist, d[q] >> bs, cls == kMove || cls == kDstOrphan);
+      lds_bin_count(hist, d[q] >> bs, cls == kMove);
       nd += cls == kDstOrphan;
       ns += cls == kSrcOrphan;
     }
@@ -212,6 +204,7 @@ __global__ __launch_bounds__(1024) void rz_dense_topscan(const uint32_t* __restr
   if (t == 0) {
     obase[nt] = carry;
     const uint32_t td = (uint32_t)carry, ts = (uint32_t)(carry >> 32);
+    obase[nt + 1] = min(td, ts);  // orphan pairs (the orphan refine's entry count, low word)
     if (td != ts) or_relaxed(&ctrl[kCtrlError], kErrOrphanMismatch);
     if (stats) st_relaxed((uint64_t*)&ctrl[kCtrlStatOrphans], (uint64_t)min(td, ts));
   }
@@ -227,7 +220,8 @@ __global__ __launch_bounds__(WG) void rz_dense_entries(const T* __restrict__ A, 
                                                         const uint32_t* __restrict__ binbase,
                                                         const uint64_t* __restrict__ obase,
                                                         uint32_t* __restrict__ ent_d, T* __restrict__ ent_v,
-                                                        T* __restrict__ src, uint32_t src_cap) {
+                                                        T* __restrict__ src, uint32_t* __restrict__ dpos,
+                                                        uint32_t src_cap) {
   constexpr int NW = WG / kWave;
   constexpr int E = MoveShared<WG, IPT>::E;
   constexpr int PER = (E + kWave - 1) / kWave;
@@ -274,7 +268,7 @@ __global__ __launch_bounds__(WG) void rz_dense_entries(const T* __restrict__ A, 
       const uint64_t md = ballot(cls == kDstOrphan);
       info[q] = cls | (lane_rank(ms) << 8) | (lane_rank(md) << 16);
       if (lane == 0) s_off[q * NW + wave] = (uint32_t)__popcll(ms) | ((uint32_t)__popcll(md) << 16);
-      rk[q] = lds_bin_add(lcnt, d[q] >> bs, cls == kMove || cls == kDstOrphan);
+      rk[q] = lds_bin_add(lcnt, d[q] >> bs, cls == kMove);
     }
     __syncthreads();
     if (wave == 0) {
@@ -310,13 +304,14 @@ __global__ __launch_bounds__(WG) void rz_dense_entries(const T* __restrict__ A, 
       if (cls == kSrcOrphan) {
         const uint32_t t = xs + carry_s + (o & 0xffffu) + ((info[q] >> 8) & 0xffu);
         if (t < src_cap) src[t] = z[q];
-      } else if (cls != kNone) {
+      } else if (cls == kDstOrphan) {
+        const uint32_t t = xd + carry_d + (o >> 16) + (info[q] >> 16);
+        if (t < src_cap) dpos[t] = d[q];
+      } else if (cls == kMove) {
         const uint32_t bin = d[q] >> bs;
         const uint32_t e = lcnt[bin] + rk[q];
-        const bool mv = cls == kMove;
-        const uint32_t t = xd + carry_d + (o >> 16) + (info[q] >> 16);
-        sd[e] = mv ? d[q] : (d[q] | kDenseOrphan);
-        sv[e] = mv ? z[q] : rank_value<T>(t);
+        sd[e] = d[q];
+        sv[e] = z[q];
         sb[e] = (uint16_t)bin;
       }
     }
@@ -377,7 +372,7 @@ __global__ __launch_bounds__(WG) void rz_dense_refine(const uint32_t* __restrict
   }
 #pragma unroll
   for (int q = 0; q < IPT; ++q) {
-    const uint32_t f = (d[q] & ~kDenseOrphan) >> bsf;
+    const uint32_t f = d[q] >> bsf;
     const bool valid = d[q] != 0xffffffffu && f < NBf;
     rk[q] = lds_bin_add(lcnt, f, valid);
     if (!valid) rk[q] = 0xffffffffu;
@@ -392,7 +387,7 @@ __global__ __launch_bounds__(WG) void rz_dense_refine(const uint32_t* __restrict
 #pragma unroll
   for (int q = 0; q < IPT; ++q) {
     if (rk[q] != 0xffffffffu) {
-      const uint32_t f = (d[q] & ~kDenseOrphan) >> bsf;
+      const uint32_t f = d[q] >> bsf;
       const uint32_t e = lcnt[f] + rk[q];
       sd[e] = d[q];
       sv[e] = v[q];
@@ -421,8 +416,7 @@ template <typename T>
 __global__ __launch_bounds__(1024) void rz_dense_scatter(T* __restrict__ A, const uint32_t* __restrict__ fent_d,
                                                         const T* __restrict__ fent_v,
                                                         const uint32_t* __restrict__ fcur, uint32_t NBf,
-                                                        uint32_t bsf, uint32_t G, const T* __restrict__ src,
-                                                        uint32_t src_cap, uint32_t sZ) {
+                                                        uint32_t bsf, uint32_t G, uint32_t sZ) {
   const uint32_t w = blockIdx.x;
   const uint32_t x = w & 7u, rest = w >> 3;
   const uint32_t g = rest % G, bin = (rest / G) * 8u + x;
@@ -432,15 +426,8 @@ __global__ __launch_bounds__(1024) void rz_dense_scatter(T* __restrict__ A, cons
   const uint32_t per = (cnt + G - 1) / G;
   const uint32_t lo = g * per, hi = min(lo + per, cnt);
   for (uint32_t q = lo + threadIdx.x; q < hi; q += 1024u) {
-    uint32_t d = fent_d[s + q];
-    T v = fent_v[s + q];
-    if (d & kDenseOrphan) {
-      const uint32_t t = value_rank<T>(v);
-      if (t >= src_cap) continue;
-      v = src[t];
-      d &= ~kDenseOrphan;
-    }
-    if (d < sZ) A[d] = v;
+    const uint32_t d = fent_d[s + q];
+    if (d < sZ) A[d] = fent_v[s + q];
   }
 }
 
@@ -628,6 +615,7 @@ static rz_status dense_launch(rz_ctx* ctx, T* A, const int32_t* R, uint32_t k, u
   uint32_t* fent_d = (uint32_t*)ctx->dense_fine.ptr;
   T* fent_v = (T*)((char*)ctx->dense_fine.ptr + ((fcap * 4 + 255) & ~(size_t)255));
   T* src = (T*)ctx->dense_src.ptr;
+  uint32_t* dpos = (uint32_t*)((char*)ctx->dense_src.ptr + (((size_t)src_cap * sizeof(T) + 255) & ~(size_t)255));
   if (d.NBt) {
     // the red-zone offsets use ent_d's space before the head writes do
     uint32_t* toff = cnt + (size_t)d.nt * d.NBt;
@@ -638,8 +626,12 @@ static rz_status dense_launch(rz_ctx* ctx, T* A, const int32_t* R, uint32_t k, u
                        (uint64_t*)nullptr, ctrl, 0);
     hipLaunchKernelGGL((rz_dense_tail_entries<WG, IPT>), dim3(d.nt), dim3(WG), lds_tail, s, R, k, sZ, k, d.L,
                        d.NBt, kTailBinLog, toff, tbinbase, ent_d);
-    hipLaunchKernelGGL(rz_dense_tail_mark, dim3(d.NBt), dim3(1024), lds_mark, s, ent_d, tbinbase, k, bits,
-                       stats ? (uint64_t*)(ctrl + kCtrlStatTail) : (uint64_t*)nullptr, d.NBt, 1u, kTailBinLog);
+    // the tail bitmap is zero here (every launch form leaves it clear), so
+    // several workgroups can share a bin and OR their words in
+    uint32_t G = (2u * (uint32_t)ctx->cu_count + d.NBt - 1) / d.NBt;
+    G = G < 1 ? 1 : (G > 16 ? 16 : G);
+    hipLaunchKernelGGL(rz_dense_tail_mark, dim3(d.NBt * G), dim3(1024), lds_mark, s, ent_d, tbinbase, k, bits,
+                       stats ? (uint64_t*)(ctrl + kCtrlStatTail) : (uint64_t*)nullptr, d.NBt, G, kTailBinLog);
   }
   uint32_t* off = cnt + (size_t)d.nt * d.NB;
   hipLaunchKernelGGL((rz_dense_count<WG, IPT>), dim3(d.nt), dim3(WG), 0, s, R, k, sZ, d.L, bits, d.NB, d.bs, cnt,
@@ -648,11 +640,14 @@ static rz_status dense_launch(rz_ctx* ctx, T* A, const int32_t* R, uint32_t k, u
   hipLaunchKernelGGL(rz_dense_topscan, dim3(1), dim3(1024), 0, s, btot, d.NB, binbase, ocnt, d.nt, obase, ctrl,
                      stats);
   hipLaunchKernelGGL((rz_dense_entries<T, WG, IPT>), dim3(d.nt), dim3(WG), lds_ent, s, A, R, k, sZ, d.L, bits, d.NB,
-                     d.bs, off, binbase, obase, ent_d, ent_v, src, src_cap);
+                     d.bs, off, binbase, obase, ent_d, ent_v, src, dpos, src_cap);
   hipLaunchKernelGGL((rz_dense_refine<T, WG, IPT>), dim3((k + S - 1) / S), dim3(WG), lds_ent, s, ent_d, ent_v,
                      binbase + d.NB, fent_d, fent_v, fcur, d.NBf, d.bsf, ctrl);
+  // orphan pairs (dpos[t], src[t]), read in rank order, into the same fine bins
+  hipLaunchKernelGGL((rz_dense_refine<T, WG, IPT>), dim3((src_cap + S - 1) / S), dim3(WG), lds_ent, s, dpos, src,
+                     (const uint32_t*)(obase + d.nt + 1), fent_d, fent_v, fcur, d.NBf, d.bsf, ctrl);
   hipLaunchKernelGGL(rz_dense_scatter<T>, dim3(d.NBf * d.G), dim3(1024), 0, s, A, fent_d, fent_v, fcur, d.NBf, d.bsf,
-                     d.G, src, src_cap, sZ);
+                     d.G, sZ);
   RZ_HIP(hipGetLastError());
   return RZ_OK;
 }
@@ -757,7 +752,9 @@ rz_status dense_moves(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, int64_t k,
     return st;
   if ((st = ensure(ctx, ctx->dense_fine, ((fcap * 4 + 255) & ~(size_t)255) + fcap * sizeof(T), s)) != RZ_OK)
     return st;
-  if ((st = ensure(ctx, ctx->dense_src, (size_t)src_cap * sizeof(T), s)) != RZ_OK) return st;
+  if ((st = ensure(ctx, ctx->dense_src, (((size_t)src_cap * sizeof(T) + 255) & ~(size_t)255) + (size_t)src_cap * 4,
+                   s)) != RZ_OK)
+    return st;
   const size_t lds = (size_t)cfg.wg_size * cfg.items * (sizeof(T) + 6) + 48 * 1024;  // + static LDS (refine)
   if (lds > (size_t)ctx->lds_max) return RZ_EINVAL;
 #define RZ_DENSE_CASE(W, I) \

@@ -233,6 +233,7 @@ def run_batched_learned(ctx, disp, lists, specs=None, batch_max_k=BATCH_MAX_K):
     side = _side_stream(ctx)
     C = None
     if len(large) and disp.trained:
+        side.wait_stream(torch.cuda.current_stream(ctx.device))  # the lists are ready on the launch stream
         with torch.cuda.stream(side):
             _, C = disp.predict(disp.features(descs.subset(large), stream=side), stream=side)
     if len(small):
