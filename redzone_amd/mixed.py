@@ -10,7 +10,7 @@ removals uniform / clustered (geometric runs, mean 64) / tail-heavy
 payloads and uniform removals, workloads.py torch code for the other two
 patterns) and processed in resident batches.
 
-    python -m redzone_amd.mixed train [--lists 2000] [--seed 1]   # writes the model
+    python -m redzone_amd.mixed train [--lists 4000] [--seed 1]   # writes the model
     python -m redzone_amd.mixed eval  [--lists 10000] [--seed 2]  # prints one JSON line
 """
 import argparse
@@ -344,7 +344,7 @@ def main():
 
     ctx = rz.Context(0)
     if args.mode == "train":
-        count, seed = args.lists or 2000, 1 if args.seed is None else args.seed
+        count, seed = args.lists or 4000, 1 if args.seed is None else args.seed
         specs = stream_specs(count, seed)
         feats, times = collect(ctx, specs, seed * 1000003, args.reps, log=log)
         disp = train_model(ctx, feats, times, epochs=args.epochs)
