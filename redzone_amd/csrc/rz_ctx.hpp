@@ -21,8 +21,9 @@ enum CtrlWord : int {
   kCtrlValidate = 192,    // validation flags: 1 = out of range, 2 = duplicate, 4 = not ascending
   kCtrlMoveEpoch = 224,   // call counter of the fused move-and-pair form (bumped by its last workgroup)
   kCtrlMoveExit = 240,    // workgroups of the fused kernel that have finished (reset by the last)
-  kCtrlMarkDone = 256,    // mark workgroups of the one-launch form that are done (reset by the last mover)
-  kCtrlWords = 288
+  kCtrlMarkDone = 256,    // mark workgroups of the one-launch form that are done (reset by the last of them)
+  kCtrlMarkReady = 288,   // 8 copies, one line apart (one per XCD): move-pair epoch + 1 once every mark is in
+  kCtrlWords = 544
 };
 
 enum ErrorBits : uint32_t {

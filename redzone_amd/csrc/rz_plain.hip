@@ -116,15 +116,16 @@ __global__ __launch_bounds__(256) void rz_pair(T* __restrict__ A, const int32_t*
 // ---------------------------------------------------------------------------
 constexpr uint64_t kStAgg = 1ull << 62, kStIncl = 2ull << 62, kStCount = (1ull << 62) - 1;
 
-// COH: the bitmap was marked in the same launch (rz_mark_move_pair): wait for
-// the nm mark workgroups after the first step's loads of R and the red zone,
-// then read bitmap words with coherent loads.
+// COH: the bitmap was marked in the same launch (rz_mark_move_pair): after the
+// first step's loads of R and the red zone, wait until the mark workgroups
+// have raised the ready word (this call's epoch + 1), then read bitmap words
+// with coherent loads.
 template <typename T, int WG, int IPT, bool COH>
 __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t* __restrict__ R, uint32_t k,
                                                uint32_t sZ, uint32_t L, uint32_t ns, uint32_t* __restrict__ bits,
                                                uint64_t* __restrict__ mstatus, uint64_t* __restrict__ slots,
                                                uint32_t slot_cap, uint32_t* __restrict__ ctrl, int stats,
-                                               uint32_t slice, uint32_t nm, uint32_t backoff) {
+                                               uint32_t slice, uint32_t backoff) {
   constexpr int NW = WG / kWave;
   constexpr int E = MoveShared<WG, IPT>::E;
   constexpr int PER = (E + kWave - 1) / kWave;
@@ -159,7 +160,8 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
       }
     }
     if constexpr (COH) {
-      if (base == a) wait_count_backoff(&ctrl[kCtrlMarkDone], nm, ctrl, backoff);  // every mark of this launch is in
+      // every mark of this launch is in: this XCD's copy of the ready word
+      if (base == a) wait_value_backoff(&ctrl[kCtrlMarkReady + 32 * (blockIdx.x & 7u)], epoch + 1u, ctrl, backoff);
 #pragma unroll
       for (int j = 0; j < IPT; ++j) {
         const uint32_t p = base + j * WG + tid;
@@ -290,7 +292,6 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
   // starts the next call's epoch
   if (tid == 0 && add_relaxed(&ctrl[kCtrlMoveExit], 1u) == ns - 1u) {
     st_relaxed(&ctrl[kCtrlMoveExit], 0u);
-    if constexpr (COH) st_relaxed(&ctrl[kCtrlMarkDone], 0u);  // every slice has passed its wait
     st_relaxed(&ctrl[kCtrlMoveEpoch], epoch + 1u);
   }
 }
@@ -301,15 +302,16 @@ __global__ __launch_bounds__(WG) void rz_move_pair(T* __restrict__ A, const int3
                                                     uint32_t* __restrict__ bits, uint64_t* __restrict__ mstatus,
                                                     uint64_t* __restrict__ slots, uint32_t slot_cap,
                                                     uint32_t* __restrict__ ctrl, int stats) {
-  move_pair_body<T, WG, IPT, false>(A, R, k, sZ, L, ns, bits, mstatus, slots, slot_cap, ctrl, stats, blockIdx.x, 0u, 0u);
+  move_pair_body<T, WG, IPT, false>(A, R, k, sZ, L, ns, bits, mstatus, slots, slot_cap, ctrl, stats, blockIdx.x, 0u);
 }
 
 // Variant 6: the mark step in the same launch.  Workgroups [0, nm) each mark
-// a tile of `tile` removals and count themselves done; the move-and-pair
-// slices come after them in dispatch order (so every mark workgroup is running
-// or done when a slice waits), prefetch their first step and wait for the count.
+// a tile of `tile` removals and count themselves done, and the last of them
+// raises a ready word per XCD; the move-and-pair slices come after them in
+// dispatch order (so every mark workgroup is running or done when a slice
+// waits), prefetch their first step and wait for their XCD's ready word.
 constexpr int kFusedMarkTileLog = 12;  // default tile: 4096 removals
-constexpr uint32_t kMarkWaitBackoff = 16;  // longest pause between polls of the mark count: 16 x 64 cycles
+constexpr uint32_t kMarkWaitBackoff = 16;  // longest pause between polls of the ready word: 16 x 64 cycles
 
 template <typename T, int WG, int IPT>
 __global__ __launch_bounds__(WG) void rz_mark_move_pair(T* __restrict__ A, const int32_t* __restrict__ R, uint32_t k,
@@ -322,16 +324,25 @@ __global__ __launch_bounds__(WG) void rz_mark_move_pair(T* __restrict__ A, const
   if (blockIdx.x < nm) {
     const uint32_t lo = blockIdx.x * tile, hi = min(lo + tile, k);
     if (!sorted || (uint32_t)R[hi - 1] >= sZ) {  // a sorted chunk below the red zone has no marks
+      // a whole tile's loads in flight at once (16 per thread for 4,096 removals and 256 threads)
       if (stats)
-        mark_body<true>(R + lo, hi - lo, k, sZ, bits, (uint64_t*)&ctrl[kCtrlStatTail], threadIdx.x, WG);
+        mark_body<true, 16>(R + lo, hi - lo, k, sZ, bits, (uint64_t*)&ctrl[kCtrlStatTail], threadIdx.x, WG);
       else
-        mark_body<false>(R + lo, hi - lo, k, sZ, bits, nullptr, threadIdx.x, WG);
+        mark_body<false, 16>(R + lo, hi - lo, k, sZ, bits, nullptr, threadIdx.x, WG);
     }
-    signal_done(&ctrl[kCtrlMarkDone]);
+    // the last mark workgroup to finish resets the count and raises the 8
+    // ready words (one per XCD, so that the waiting slices poll 8 lines)
+    wait_all_memory();
+    __syncthreads();
+    if (threadIdx.x == 0 && add_relaxed(&ctrl[kCtrlMarkDone], 1u) == nm - 1u) {
+      st_relaxed(&ctrl[kCtrlMarkDone], 0u);
+      const uint32_t ready = ld_relaxed(&ctrl[kCtrlMoveEpoch]) + 1u;  // no slice can have finished yet
+      for (int x = 0; x < 8; ++x) st_relaxed(&ctrl[kCtrlMarkReady + 32 * x], ready);
+    }
     return;
   }
   move_pair_body<T, WG, IPT, true>(A, R, k, sZ, L, ns, bits, mstatus, slots, slot_cap, ctrl, stats, blockIdx.x - nm,
-                                   nm, backoff);
+                                   backoff);
 }
 
 // ---------------------------------------------------------------------------

@@ -45,11 +45,11 @@ template <bool COH, typename T> __device__ __forceinline__ T ld_x(const T* p) {
 // 1. mark: R[0, count) (a window of the removal list of a list with k
 // removals and red zone start sZ); thread gtid of `stride` threads.
 // ---------------------------------------------------------------------------
-template <bool STATS>
+// U: removals per thread whose loads are in flight together.
+template <bool STATS, int U = 4>
 __device__ __forceinline__ void mark_body(const int32_t* __restrict__ R, uint32_t count, uint32_t k, uint32_t sZ,
                                           uint32_t* __restrict__ bits, uint64_t* __restrict__ stat_tail,
                                           uint32_t gtid, uint32_t stride) {
-  constexpr int U = 4;
   uint32_t hits = 0;
   for (uint32_t i0 = gtid; i0 < count; i0 += stride * U) {
     uint32_t r[U];
@@ -297,13 +297,14 @@ __device__ __forceinline__ void wait_count(const uint32_t* counter, uint32_t tar
   __syncthreads();
 }
 
-// As wait_count, with a pause that doubles from 64 to 64*cap cycles between
-// polls, so that many waiting workgroups do not crowd the counter's L2 channel.
-__device__ __forceinline__ void wait_count_backoff(const uint32_t* counter, uint32_t target, uint32_t* ctrl,
+// Wait (thread 0 polls, the workgroup waits at a barrier) until *word ==
+// value, with a pause that doubles from 64 to 64*cap cycles between polls; a
+// bounded wait that flags kErrBarrierTimeout if it gives up.
+__device__ __forceinline__ void wait_value_backoff(const uint32_t* word, uint32_t value, uint32_t* ctrl,
                                                    uint32_t cap) {
   if (threadIdx.x == 0) {
     uint32_t spins = 0, pause = 1;
-    while (ld_relaxed(counter) < target) {
+    while (ld_relaxed(word) != value) {
       for (uint32_t i = 0; i < pause; ++i) __builtin_amdgcn_s_sleep(1);
       if (pause < cap) pause <<= 1;
       if (++spins > (1u << 22)) {
