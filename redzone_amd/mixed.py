@@ -214,10 +214,12 @@ def run_batched_learned(ctx, disp, lists, specs=None, batch_max_k=BATCH_MAX_K):
     """Lists below batch_max_k removals in one persistent launch (Redzone);
     the others launched on their own with the method the MLP picks.  The
     features and the forward pass of the large lists run on a second stream
-    while the batch runs, and only the choice comes back to the host.  Wall
-    time (s) of everything including the decision, the descriptor upload and
-    the batch plan; returns (wall, method that ran per list).
-    lists: [(data, removals), ...] or a ListDescs."""
+    while the batch runs, and only the choice comes back to the host.  (The
+    large lists' own launches were also tried on that stream, beside the
+    batch, through a second context: no gain, the two share the memory
+    system.)  Wall time (s) of everything including the decision, the
+    descriptor upload and the batch plan; returns (wall, method that ran per
+    list).  lists: [(data, removals), ...] or a ListDescs."""
     import torch
 
     from .dispatch import rule_choice
@@ -232,8 +234,9 @@ def run_batched_learned(ctx, disp, lists, specs=None, batch_max_k=BATCH_MAX_K):
     t0 = time.perf_counter()
     side = _side_stream(ctx)
     C = None
-    if len(large) and disp.trained:
+    if len(large):
         side.wait_stream(torch.cuda.current_stream(ctx.device))  # the lists are ready on the launch stream
+    if len(large) and disp.trained:
         with torch.cuda.stream(side):
             _, C = disp.predict(disp.features(descs.subset(large), stream=side), stream=side)
     if len(small):
