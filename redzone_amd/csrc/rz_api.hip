@@ -128,7 +128,8 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
   DeviceBuffer* bufs[] = {&ctx->tail_bits, &ctx->list_bits, &ctx->counts, &ctx->orphans,
                           &ctx->status,    &ctx->ctrl,      &ctx->queue,  &ctx->mstatus, &ctx->slots,
                           &ctx->dense_cnt, &ctx->dense_small, &ctx->dense_ent, &ctx->dense_fine,
-                          &ctx->dense_src, &ctx->dispatch, &ctx->gstatus};
+                          &ctx->dense_src, &ctx->dispatch, &ctx->gstatus, &ctx->ploaded, &ctx->pgcnt,
+                          &ctx->pgdone};
   for (DeviceBuffer* b : bufs)
     if (b->ptr) (void)hipFree(b->ptr);
   if (ctx->ctrl_host) (void)hipHostFree(ctx->ctrl_host);

@@ -57,6 +57,9 @@ struct rz_ctx {
   rz::DeviceBuffer counts;     // per-slice orphan counts (uint64: dest | src << 32)
   rz::DeviceBuffer orphans;    // dest orphans when RZ_PRESERVE_REMOVALS is set
   rz::DeviceBuffer status;     // look-back status words of the partition
+  rz::DeviceBuffer ploaded;    // partition: per tile, the epoch once its items are loaded
+  rz::DeviceBuffer pgcnt;      // partition: per 64 tiles, the number of them loaded (zero between calls)
+  rz::DeviceBuffer pgdone;     // partition: per 64 tiles, the epoch once all of them are loaded
   rz::DeviceBuffer ctrl;       // control block (rz::CtrlWord)
   rz::DeviceBuffer queue;      // ticket / exit / done words of the single-list queue launch
   rz::DeviceBuffer mstatus;    // fused form: two halves of kMaxSlices look-back words, used alternately

@@ -20,9 +20,15 @@
 //                 end of its own input, so the in-place pass never overwrites
 //                 an item before it was read.  It clears the bitmap words it
 //                 read, so the bitmap is zero again for the next call.
-//                 The 16-byte aligned form stages the kept items in LDS and
-//                 writes them as whole vectors, and waits on the nearest
-//                 predecessor's status word before it scans a window.
+//                 The 16-byte aligned form (rz_compact_vec) publishes a
+//                 tile's kept count as soon as it has the tile's bitmap words
+//                 (the count needs no item), so the look-back never waits for
+//                 loads; the in-place guard is separate: a tile sets its
+//                 loaded flag (and its group of 64 tiles a group word) once
+//                 its items are in registers, and a tile writes only after
+//                 every tile overlapping its output range has loaded.  It
+//                 stages the kept items in LDS and writes them as whole
+//                 vectors.
 #include "rz_ctx.hpp"
 #include "rz_device.hpp"
 
@@ -214,7 +220,9 @@ struct CompactShared {
 
 template <typename T, int WG, int VPT>
 __global__ __launch_bounds__(WG) void rz_compact_vec(T* __restrict__ A, uint32_t n, uint32_t* __restrict__ bits,
-                                                      uint64_t* __restrict__ status, uint32_t* __restrict__ ctrl) {
+                                                      uint64_t* __restrict__ status, uint32_t* __restrict__ loaded,
+                                                      uint32_t* __restrict__ gcnt, uint32_t* __restrict__ gdone,
+                                                      uint32_t ntiles, uint32_t* __restrict__ ctrl) {
   constexpr int V = 16 / sizeof(T);
   constexpr int NW = WG / kWave;
   constexpr int E = VPT * NW;
@@ -240,17 +248,23 @@ __global__ __launch_bounds__(WG) void rz_compact_vec(T* __restrict__ A, uint32_t
     sh.bits[w] = v;
   }
   uint4 x[VPT];
+  auto load_items = [&]() {
 #pragma unroll
-  for (int j = 0; j < VPT; ++j) {
-    const uint32_t p0 = base + (uint32_t)(j * WG + tid) * V;
-    if (p0 + V <= n) {
-      x[j] = *(const uint4*)(A + p0);
-    } else {
-      x[j] = make_uint4(0u, 0u, 0u, 0u);
-      load_partial<T, V>(A, p0, n, x[j]);
+    for (int j = 0; j < VPT; ++j) {
+      const uint32_t p0 = base + (uint32_t)(j * WG + tid) * V;
+      if (p0 + V <= n) {
+        x[j] = *(const uint4*)(A + p0);
+      } else {
+        x[j] = make_uint4(0u, 0u, 0u, 0u);
+        load_partial<T, V>(A, p0, n, x[j]);
+      }
     }
-  }
+  };
+  // wave 0 loads its items after it has published the tile's count, so that
+  // its status store does not wait behind them (loads and stores retire in order)
+  if (wave != 0) load_items();
   __syncthreads();
+  // the tile's kept count depends on the bitmap alone
   uint32_t keep[VPT], pre[VPT];
 #pragma unroll
   for (int j = 0; j < VPT; ++j) {
@@ -263,7 +277,6 @@ __global__ __launch_bounds__(WG) void rz_compact_vec(T* __restrict__ A, uint32_t
     pre[j] = keep_count_prefix((uint32_t)__popc(km), &tot);
     if (lane == 0) sh.off[j * NW + wave] = tot;
   }
-  wait_all_memory();  // every item load of this tile has returned
   __syncthreads();
   if (wave == 0) {
     uint32_t loc[PER];
@@ -283,21 +296,37 @@ __global__ __launch_bounds__(WG) void rz_compact_vec(T* __restrict__ A, uint32_t
       ex += loc[q];
     }
     if (lane == kWave - 1) sh.off[E] = xs;
+    const uint32_t agg = __shfl(xs, kWave - 1, kWave);
+    // the count goes out before any item is read: the offsets of later
+    // tiles never wait for loads
+    if (lane == 0) st_relaxed(&status[t], pack_status(t == 0 ? kFlagIncl : kFlagAgg, epoch, agg));
+    load_items();
   }
+  wait_all_memory();  // every item load of this tile has returned
   __syncthreads();
+  if (tid == 0) {
+    // this tile's input may now be overwritten: its flag, and its group's
+    // word once all 64 tiles of the group (fewer in the last) have loaded.
+    // The counters and the words live in separate arrays: words polled next
+    // to counters being incremented slowed the pass by half.
+    st_relaxed(&loaded[t], epoch);
+    const uint32_t g = t >> 6, members = min(64u, ntiles - (g << 6));
+    if (add_relaxed(&gcnt[g], 1u) == members - 1u) {
+      st_relaxed(&gcnt[g], 0u);
+      st_relaxed(&gdone[g], epoch);
+    }
+  }
   // kept items in tile order into LDS (wave 0 stages its items, then looks back)
 #pragma unroll
   for (int j = 0; j < VPT; ++j) stage_kept<T, V>(sh.stage, x[j], keep[j], sh.off[j * NW + wave] + pre[j]);
   if (wave == 0) {
     const uint32_t agg = sh.off[E];
     uint32_t excl = 0;
-    if (t == 0) {
-      if (lane == 0) st_relaxed(&status[0], pack_status(kFlagIncl, epoch, agg));
-    } else {
-      if (lane == 0) st_relaxed(&status[t], pack_status(kFlagAgg, epoch, agg));
-      // Wait on the nearest predecessor's word alone first (one load per poll,
-      // pause doubling to 16 x 64 cycles): polling whole windows from every
-      // waiting tile costs more fabric bandwidth than the moves.
+    if (t != 0) {
+      // The nearest predecessor's word alone first (one load per poll, pause
+      // doubling to 16 x 64 cycles), then windows of 64 back to an inclusive
+      // prefix: polling whole windows from every waiting tile costs more
+      // fabric bandwidth than the moves.
       uint32_t pause = 1, spins = 0;
       while (true) {
         const uint64_t st = ld_relaxed(&status[t - 1]);
@@ -331,6 +360,37 @@ __global__ __launch_bounds__(WG) void rz_compact_vec(T* __restrict__ A, uint32_t
         jb -= kWave;
       }
       if (lane == 0) st_relaxed(&status[t], pack_status(kFlagIncl, epoch, excl + agg));
+      // The pass is in place: every tile whose input overlaps [excl, base)
+      // must have loaded before this tile writes.  Those are tiles [u, t):
+      // whole groups of 64 by their word, the two ends tile by tile.
+      const uint32_t u = excl / TILE;
+      const uint32_t g0 = (u + 63u) >> 6, g1 = t >> 6;
+      const bool groups = g0 < g1;
+      const uint32_t a_end = groups ? (g0 << 6) : t, b_beg = groups ? (g1 << 6) : t;
+      const uint32_t na = a_end - u, ng = groups ? g1 - g0 : 0u, nb = t - b_beg;
+      const uint32_t total = na + ng + nb;
+      spins = 0;
+      pause = 1;
+      for (uint32_t c0 = 0; c0 < total;) {
+        const uint32_t c = c0 + lane;
+        bool ok = true;
+        if (c < na)
+          ok = ld_relaxed(&loaded[u + c]) == epoch;
+        else if (c < na + ng)
+          ok = ld_relaxed(&gdone[g0 + (c - na)]) == epoch;
+        else if (c < total)
+          ok = ld_relaxed(&loaded[b_beg + (c - na - ng)]) == epoch;
+        if (ballot(!ok)) {
+          if (++spins > (1u << 22)) {
+            if (lane == 0) or_relaxed(&ctrl[kCtrlError], kErrLookbackTimeout);
+            break;
+          }
+          for (uint32_t i = 0; i < pause; ++i) __builtin_amdgcn_s_sleep(1);
+          if (pause < 16u) pause <<= 1;
+          continue;
+        }
+        c0 += kWave;
+      }
     }
     if (lane == 0) sh.excl = excl;
   }
@@ -427,6 +487,12 @@ static rz_status partition_impl(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, 
   rz_status st;
   if ((st = ensure(ctx, ctx->list_bits, (size_t)((n + 63) / 64) * 8, s)) != RZ_OK) return st;
   if ((st = ensure(ctx, ctx->status, (size_t)tiles * 8, s)) != RZ_OK) return st;
+  const int64_t groups = (tiles + 63) / 64;
+  if (vec) {
+    if ((st = ensure(ctx, ctx->ploaded, (size_t)tiles * 4, s)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->pgcnt, (size_t)groups * 4, s)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->pgdone, (size_t)groups * 4, s)) != RZ_OK) return st;
+  }
   uint32_t* bits = (uint32_t*)ctx->list_bits.ptr;
   uint32_t* ctrl = (uint32_t*)ctx->ctrl.ptr;
   const int64_t gw = (k + 255) / 256;
@@ -445,7 +511,8 @@ static rz_status partition_impl(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, 
   if ((st = prof_mark(ctx, flags, 1, s)) != RZ_OK) return st;
   if (vec)
     hipLaunchKernelGGL((rz_compact_vec<T, kVecWG, VecShape<T>::VPT>), dim3((uint32_t)tiles), dim3(kVecWG), 0, s, A,
-                       (uint32_t)n, bits, (uint64_t*)ctx->status.ptr, ctrl);
+                       (uint32_t)n, bits, (uint64_t*)ctx->status.ptr, (uint32_t*)ctx->ploaded.ptr,
+                       (uint32_t*)ctx->pgcnt.ptr, (uint32_t*)ctx->pgdone.ptr, (uint32_t)tiles, ctrl);
   else
     hipLaunchKernelGGL((rz_compact<T, kPartWG, kPartIPT>), dim3((uint32_t)tiles), dim3(kPartWG), 0, s, A,
                        (uint32_t)n, bits, (uint64_t*)ctx->status.ptr, ctrl);

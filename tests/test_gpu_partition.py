@@ -99,3 +99,17 @@ def test_partition_bucketed_marks(ctx, oracle, esz):
                           (1 << 23, (1 << 23) - 1, "uniform"), ((3 << 22) + 1, 1 << 22, "tail_heavy")]:
         announce("partition bucketed", esz, n, k, pattern)
         _run(ctx, oracle, W.payload(esz, n, seed=40), W.removals(pattern, n, k, seed=41))
+
+
+@pytest.mark.parametrize("esz", [4, 8, 16])
+def test_partition_tile_counts_change_between_calls(ctx, oracle, esz):
+    """Back-to-back calls whose tile counts grow and shrink (partial groups of
+    64 tiles at the end, groups that a previous call left in other states):
+    the per-tile loaded flags and the per-group words are keyed by call epoch."""
+    tile = 1024 * 4 * (16 // esz)
+    rng = np.random.default_rng(60 + esz)
+    for i, tiles in enumerate([197, 70, 331, 5, 130, 64, 129]):
+        n = tiles * tile - int(rng.integers(0, tile // 2))
+        for frac in (0.01, 0.5):
+            announce("partition tiles", esz, tiles, n, frac)
+            _run(ctx, oracle, W.payload(esz, n, seed=61 + i), W.uniform_removals(n, int(n * frac), seed=62 + i))
