@@ -45,6 +45,8 @@
 // the entries kernel as it reads it.  With an invalid list the passes still
 // classify identically, every index is checked against its buffer, and the
 // orphan-count mismatch (or a fine-bin overflow) is flagged.
+#include <atomic>
+
 #include "rz_redzone_dev.hpp"
 
 namespace rz {
@@ -576,6 +578,15 @@ static uint32_t bin_log(int64_t span, int64_t target) {
   return lg;
 }
 
+// Dynamic-LDS limits are set with hipFuncSetAttribute once per device (the
+// current one, which every entry point selects): bit d of the mask.
+static bool attr_needed(const std::atomic<uint64_t>& mask, int dev) {
+  return dev < 0 || dev >= 64 || !(mask.load(std::memory_order_relaxed) & (1ull << dev));
+}
+static void attr_done(std::atomic<uint64_t>& mask, int dev) {
+  if (dev >= 0 && dev < 64) mask.fetch_or(1ull << dev, std::memory_order_relaxed);
+}
+
 // small buffer: [btot | binbase (+1) | tbtot | tbinbase (+1) | fcur] (uint32,
 // kSmallStride each, fcur kFineMax + 64), then [ocnt | obase (+1)] (uint64)
 constexpr size_t kSmallStride = kCoarseMax + 64;
@@ -589,8 +600,8 @@ static rz_status dense_launch(rz_ctx* ctx, T* A, const int32_t* R, uint32_t k, u
   const size_t lds_ent = (size_t)S * (sizeof(T) + 4 + 2);
   const size_t lds_tail = (size_t)S * (4 + 2);
   const size_t lds_mark = (size_t)4 << (kTailBinLog - 5);
-  static bool attr_set = false;
-  if (!attr_set) {
+  static std::atomic<uint64_t> attr_mask{0};
+  if (attr_needed(attr_mask, ctx->device)) {
     RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_entries<T, WG, IPT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds_ent));
     RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_refine<T, WG, IPT>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -599,7 +610,7 @@ static rz_status dense_launch(rz_ctx* ctx, T* A, const int32_t* R, uint32_t k, u
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_tail));
     RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_tail_mark, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)kTailMarkLdsMax));
-    attr_set = true;
+    attr_done(attr_mask, ctx->device);
   }
   uint32_t* cnt = (uint32_t*)ctx->dense_cnt.ptr;
   uint32_t* btot = (uint32_t*)ctx->dense_small.ptr;
@@ -681,13 +692,13 @@ rz_status bucket_marks(rz_ctx* ctx, const int32_t* R, int64_t k, uint32_t base, 
   const size_t lds_tail = (size_t)S * (4 + 2);
   const size_t lds_mark = (size_t)4 << (kBucketBinLog - 5);
   if ((size_t)ctx->lds_max < lds_mark || (size_t)ctx->lds_max < lds_tail + 16 * 1024) return RZ_EINVAL;
-  static bool attr_set = false;
-  if (!attr_set) {
+  static std::atomic<uint64_t> attr_mask{0};
+  if (attr_needed(attr_mask, ctx->device)) {
     RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_tail_entries<WG, IPT>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_tail));
     RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_tail_mark, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)kTailMarkLdsMax));
-    attr_set = true;
+    attr_done(attr_mask, ctx->device);
   }
   uint32_t* cnt = (uint32_t*)ctx->dense_cnt.ptr;
   uint32_t* toff = cnt + (size_t)nt * NBt;
