@@ -255,6 +255,7 @@ __global__ __launch_bounds__(WG) void rz_dense_entries(const T* __restrict__ A, 
     for (int q = 0; q < IPT; ++q) {
       const uint32_t p = base + q * WG + tid;
       d[q] = sZ;
+      z[q] = T();  // defined in every lane: keeps 16-byte items in registers
       wv[q] = 0xffffffffu;
       if (p < b) {
         d[q] = (uint32_t)R[p];

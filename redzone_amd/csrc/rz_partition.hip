@@ -70,6 +70,7 @@ __global__ __launch_bounds__(WG) void rz_compact(T* __restrict__ A, uint32_t n, 
   for (int j = 0; j < IPT; ++j) {
     const uint32_t p = base + j * WG + tid;
     uint32_t w = 0xffffffffu;
+    x[j] = T();  // defined in every lane: keeps 16-byte items in registers
     if (p < n) {
       x[j] = A[p];
       w = bits[p >> 5];

@@ -143,22 +143,28 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
     for (uint32_t w = slice * WG + tid; w < (uint32_t)kMaxSlices; w += ns * WG) other[w] = 0ull;
   }
 
+  // one step's loads: R and the red zone, and the bitmap words (coherent
+  // loads in the one-launch form, once the marks are in)
+  auto load_rz = [&](uint32_t base_, uint32_t (&dd)[IPT], T (&zz)[IPT], uint32_t (&ww)[IPT], bool with_bits) {
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t p = base_ + j * WG + tid;
+      if (p < b) {
+        dd[j] = (uint32_t)R[p];
+        zz[j] = Z[p];
+        ww[j] = with_bits ? (COH ? ld_relaxed(&bits[p >> 5]) : bits[p >> 5]) : 0u;
+      } else {
+        dd[j] = sZ;           // invalid destination
+        zz[j] = T();          // (a defined value keeps 16-byte items in registers)
+        ww[j] = 0xffffffffu;  // dead source
+      }
+    }
+  };
   uint32_t carry_s = 0, carry_d = 0;
   for (uint32_t base = a; base < b; base += WG * IPT) {
     uint32_t d[IPT], wv[IPT], info[IPT];
     T z[IPT];
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const uint32_t p = base + j * WG + tid;
-      if (p < b) {
-        d[j] = (uint32_t)R[p];
-        z[j] = Z[p];
-        if constexpr (!COH) wv[j] = bits[p >> 5];
-      } else {
-        d[j] = sZ;           // invalid destination
-        wv[j] = 0xffffffffu; // dead source
-      }
-    }
+    load_rz(base, d, z, wv, !COH);
     if constexpr (COH) {
       // every mark of this launch is in: this XCD's copy of the ready word
       if (base == a) wait_value_backoff(&ctrl[kCtrlMarkReady + 32 * (blockIdx.x & 7u)], epoch + 1u, ctrl, backoff);
@@ -519,7 +525,7 @@ static bool dispatch_mark_move_pair(int wg, int ipt, hipStream_t s, T* A, const 
 #define RZ_MMP_CASE(W, I)                                                                                      \
   if (wg == W && ipt == I) {                                                                                   \
     hipLaunchKernelGGL((rz_mark_move_pair<T, W, I>), dim3(nm + ns), dim3(W), lds, s, A, R, k, sZ, L, ns, bits, \
-                       mstatus, slots, slot_cap, ctrl, stats, nm, tile, bo, sorted);                            \
+                       mstatus, slots, slot_cap, ctrl, stats, nm, tile, bo, sorted);                           \
     return true;                                                                                               \
   }
   RZ_MMP_CASE(256, 2) RZ_MMP_CASE(256, 4) RZ_MMP_CASE(256, 8) RZ_MMP_CASE(256, 16)
