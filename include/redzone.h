@@ -53,6 +53,12 @@ typedef enum rz_status {
 #define RZ_PROFILE 0x10u          /* time each kernel with events on the stream; read with rz_last_profile */
 #define RZ_SORTED 0x20u           /* d_removals is strictly ascending: the mark step reads only its red-zone suffix
                                      (checked by RZ_VALIDATE; if it is not true the result is unspecified) */
+#define RZ_ANY_PAIRING 0x40u      /* the fused forms (variants 4 and 6) may pair orphans inside a slice first
+                                     (paper p.14, early pairing): the same survivors, every kept item outside the
+                                     tail still in place, but which tail item fills which hole may differ from
+                                     the canonical array (redzone.hxx:913-934 pairs by global rank).  Other forms
+                                     return the canonical array, which also meets this contract.  With RZ_STATS
+                                     the orphan count is that of the pairs left to the global exchange. */
 
 typedef struct rz_ctx rz_ctx;
 

@@ -15,11 +15,11 @@ import os
 
 from . import _native
 from ._native import (RZ_COOPERATIVE, RZ_METHOD_DENSE, RZ_METHOD_ORDERED, RZ_METHOD_PARTITION, RZ_METHOD_REDZONE,
-                      RZ_PRESERVE_REMOVALS, RZ_PROFILE, RZ_SORTED, RZ_STATS, RZ_VALIDATE, BatchInfo, ListDesc,
+                      RZ_ANY_PAIRING, RZ_PRESERVE_REMOVALS, RZ_PROFILE, RZ_SORTED, RZ_STATS, RZ_VALIDATE, BatchInfo, ListDesc,
                       RedzoneError, Stats, TileConfig)
 
 __all__ = ["Context", "Batch", "ListDescs", "remove", "partition", "TileConfig", "Stats", "RedzoneError", "RZ_PRESERVE_REMOVALS",
-           "RZ_VALIDATE", "RZ_COOPERATIVE", "RZ_STATS", "RZ_PROFILE", "RZ_SORTED", "RZ_METHOD_REDZONE",
+           "RZ_VALIDATE", "RZ_COOPERATIVE", "RZ_STATS", "RZ_PROFILE", "RZ_SORTED", "RZ_ANY_PAIRING", "RZ_METHOD_REDZONE",
            "RZ_METHOD_ORDERED", "RZ_METHOD_DENSE", "RZ_METHOD_PARTITION", "elem_bytes_of", "default_context"]
 
 

@@ -18,6 +18,7 @@ RZ_COOPERATIVE = 0x4
 RZ_STATS = 0x8
 RZ_PROFILE = 0x10
 RZ_SORTED = 0x20
+RZ_ANY_PAIRING = 0x40
 # dispatcher methods (RZ_METHOD_*)
 RZ_METHOD_REDZONE, RZ_METHOD_ORDERED, RZ_METHOD_DENSE, RZ_METHOD_PARTITION = range(4)
 

@@ -125,7 +125,7 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
                                                uint32_t sZ, uint32_t L, uint32_t ns, uint32_t* __restrict__ bits,
                                                uint64_t* __restrict__ mstatus, uint64_t* __restrict__ slots,
                                                uint32_t slot_cap, uint32_t* __restrict__ ctrl, int stats,
-                                               uint32_t slice, uint32_t backoff) {
+                                               int early, uint32_t slice, uint32_t backoff) {
   constexpr int NW = WG / kWave;
   constexpr int E = MoveShared<WG, IPT>::E;
   constexpr int PER = (E + kWave - 1) / kWave;
@@ -207,7 +207,10 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
         s_off[E] = x;
         if (base + WG * IPT >= b) {
           // the slice's counts are final: publish them before the last moves
-          const uint64_t agg = ((uint64_t)(carry_d + (x >> 16)) << 31) | (uint64_t)(carry_s + (x & 0xffffu));
+          // (with early pairing, only the orphans left after the slice pairs its own)
+          const uint32_t td = carry_d + (x >> 16), ts = carry_s + (x & 0xffffu);
+          const uint32_t m = early ? min(td, ts) : 0u;
+          const uint64_t agg = ((uint64_t)(td - m) << 31) | (uint64_t)(ts - m);
           st_relaxed(&st[slice], (slice == 0 ? kStIncl : kStAgg) | agg);
         }
       }
@@ -231,9 +234,17 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
     __syncthreads();
   }
 
+  // Early pairing (RZ_ANY_PAIRING): the slice's first m dest orphans take its
+  // first m source orphans here, and only the rest go to the global exchange.
+  const uint32_t m = early ? min(carry_d, carry_s) : 0u;
+  for (uint32_t j = tid; j < m; j += WG) {
+    const uint32_t dst = s_orph[j], src = s_orph[L - 1u - j];
+    if (dst < sZ && src < k) A[dst] = Z[src];
+  }
+  const uint32_t rd = carry_d - m, rs = carry_s - m;
   // exclusive orphan counts of the earlier slices
   if (wave == 0) {
-    const uint64_t agg = ((uint64_t)carry_d << 31) | carry_s;
+    const uint64_t agg = ((uint64_t)rd << 31) | rs;
     uint64_t excl = 0;
     if (slice > 0) {
       int64_t j = (int64_t)slice - 1;
@@ -276,11 +287,11 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
   const uint64_t excl = s_excl;
   const uint32_t xd = (uint32_t)(excl >> 31), xs = (uint32_t)excl & 0x7fffffffu;
   const uint64_t tag = (uint64_t)epoch << 32;
-  const uint32_t norph = carry_d + carry_s;
+  const uint32_t norph = rd + rs;
   for (uint32_t e = tid; e < norph; e += WG) {
-    const bool is_d = e < carry_d;
-    const uint32_t mine = is_d ? s_orph[e] : s_orph[L - 1u - (e - carry_d)];
-    const uint32_t t = is_d ? xd + e : xs + (e - carry_d);
+    const bool is_d = e < rd;
+    const uint32_t mine = is_d ? s_orph[m + e] : s_orph[L - 1u - (m + e - rd)];
+    const uint32_t t = is_d ? xd + e : xs + (e - rd);
     if (t >= slot_cap) continue;  // only with an invalid removal list
     const uint64_t old = xchg_relaxed(&slots[t], tag | (uint64_t)(mine + 1u));
     if ((uint32_t)(old >> 32) != epoch || (uint32_t)old == 0u) continue;  // first to arrive
@@ -307,8 +318,9 @@ __global__ __launch_bounds__(WG) void rz_move_pair(T* __restrict__ A, const int3
                                                     uint32_t sZ, uint32_t L, uint32_t ns,
                                                     uint32_t* __restrict__ bits, uint64_t* __restrict__ mstatus,
                                                     uint64_t* __restrict__ slots, uint32_t slot_cap,
-                                                    uint32_t* __restrict__ ctrl, int stats) {
-  move_pair_body<T, WG, IPT, false>(A, R, k, sZ, L, ns, bits, mstatus, slots, slot_cap, ctrl, stats, blockIdx.x, 0u);
+                                                    uint32_t* __restrict__ ctrl, int stats, int early) {
+  move_pair_body<T, WG, IPT, false>(A, R, k, sZ, L, ns, bits, mstatus, slots, slot_cap, ctrl, stats, early,
+                                    blockIdx.x, 0u);
 }
 
 // Variant 6: the mark step in the same launch.  Workgroups [0, nm) each mark
@@ -326,7 +338,7 @@ __global__ __launch_bounds__(WG) void rz_mark_move_pair(T* __restrict__ A, const
                                                          uint64_t* __restrict__ mstatus,
                                                          uint64_t* __restrict__ slots, uint32_t slot_cap,
                                                          uint32_t* __restrict__ ctrl, int stats, uint32_t nm,
-                                                         uint32_t tile, uint32_t backoff, int sorted) {
+                                                         uint32_t tile, uint32_t backoff, int sorted, int early) {
   if (blockIdx.x < nm) {
     const uint32_t lo = blockIdx.x * tile, hi = min(lo + tile, k);
     if (!sorted || (uint32_t)R[hi - 1] >= sZ) {  // a sorted chunk below the red zone has no marks
@@ -347,8 +359,8 @@ __global__ __launch_bounds__(WG) void rz_mark_move_pair(T* __restrict__ A, const
     }
     return;
   }
-  move_pair_body<T, WG, IPT, true>(A, R, k, sZ, L, ns, bits, mstatus, slots, slot_cap, ctrl, stats, blockIdx.x - nm,
-                                   backoff);
+  move_pair_body<T, WG, IPT, true>(A, R, k, sZ, L, ns, bits, mstatus, slots, slot_cap, ctrl, stats, early,
+                                   blockIdx.x - nm, backoff);
 }
 
 // ---------------------------------------------------------------------------
@@ -498,12 +510,12 @@ static bool dispatch_move(int wg, int ipt, dim3 grid, hipStream_t s, T* A, const
 template <typename T>
 static bool dispatch_move_pair(int wg, int ipt, hipStream_t s, T* A, const int32_t* R, uint32_t k, uint32_t sZ,
                                uint32_t L, uint32_t ns, uint32_t* bits, uint64_t* mstatus, uint64_t* slots,
-                               uint32_t slot_cap, uint32_t* ctrl, int stats) {
+                               uint32_t slot_cap, uint32_t* ctrl, int stats, int early) {
   const size_t lds = (size_t)L * 4;
 #define RZ_MP_CASE(W, I)                                                                                     \
   if (wg == W && ipt == I) {                                                                                 \
     hipLaunchKernelGGL((rz_move_pair<T, W, I>), dim3(ns), dim3(W), lds, s, A, R, k, sZ, L, ns, bits, mstatus, \
-                       slots, slot_cap, ctrl, stats);                                                        \
+                       slots, slot_cap, ctrl, stats, early);                                                 \
     return true;                                                                                             \
   }
   RZ_MP_CASE(256, 2) RZ_MP_CASE(256, 4) RZ_MP_CASE(256, 8) RZ_MP_CASE(256, 16)
@@ -516,7 +528,8 @@ static bool dispatch_move_pair(int wg, int ipt, hipStream_t s, T* A, const int32
 template <typename T>
 static bool dispatch_mark_move_pair(int wg, int ipt, hipStream_t s, T* A, const int32_t* R, uint32_t k, uint32_t sZ,
                                     uint32_t L, uint32_t ns, uint32_t* bits, uint64_t* mstatus, uint64_t* slots,
-                                    uint32_t slot_cap, uint32_t* ctrl, int stats, int tile_log, int sorted) {
+                                    uint32_t slot_cap, uint32_t* ctrl, int stats, int tile_log, int sorted,
+                                    int early) {
   const uint32_t bo = kMarkWaitBackoff;
   const size_t lds = (size_t)L * 4;
   tile_log = tile_log == 0 ? kFusedMarkTileLog : (tile_log < 8 ? 8 : (tile_log > 16 ? 16 : tile_log));
@@ -525,7 +538,7 @@ static bool dispatch_mark_move_pair(int wg, int ipt, hipStream_t s, T* A, const 
 #define RZ_MMP_CASE(W, I)                                                                                      \
   if (wg == W && ipt == I) {                                                                                   \
     hipLaunchKernelGGL((rz_mark_move_pair<T, W, I>), dim3(nm + ns), dim3(W), lds, s, A, R, k, sZ, L, ns, bits, \
-                       mstatus, slots, slot_cap, ctrl, stats, nm, tile, bo, sorted);                           \
+                       mstatus, slots, slot_cap, ctrl, stats, nm, tile, bo, sorted, early);                    \
     return true;                                                                                               \
   }
   RZ_MMP_CASE(256, 2) RZ_MMP_CASE(256, 4) RZ_MMP_CASE(256, 8) RZ_MMP_CASE(256, 16)
@@ -643,7 +656,8 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
   if (one_launch) {
     if (!dispatch_mark_move_pair<T>(cfg.wg_size, cfg.items, s, A, R, (uint32_t)k, sZ, L, ns, bits,
                                     (uint64_t*)ctx->mstatus.ptr, (uint64_t*)ctx->slots.ptr, (uint32_t)(k / 2 + 1),
-                                    ctrl, stats, cfg.slice_steps, (flags & RZ_SORTED) ? 1 : 0))
+                                    ctrl, stats, cfg.slice_steps, (flags & RZ_SORTED) ? 1 : 0,
+                                    (flags & RZ_ANY_PAIRING) ? 1 : 0))
       return RZ_EINVAL;
     RZ_HIP(hipGetLastError());
     if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;
@@ -652,7 +666,7 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
   if (fused) {
     if (!dispatch_move_pair<T>(cfg.wg_size, cfg.items, s, A, R, (uint32_t)k, sZ, L, ns, bits,
                                (uint64_t*)ctx->mstatus.ptr, (uint64_t*)ctx->slots.ptr, (uint32_t)(k / 2 + 1), ctrl,
-                               stats))
+                               stats, (flags & RZ_ANY_PAIRING) ? 1 : 0))
       return RZ_EINVAL;
     RZ_HIP(hipGetLastError());
     if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;

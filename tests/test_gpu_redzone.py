@@ -410,3 +410,37 @@ def test_premarked(ctx, oracle, esz):
     assert ctx.remove_premarked(A, R, bits) == 0
     torch.cuda.synchronize()
     assert int(bits.abs().sum().item()) == 0
+
+
+@pytest.mark.parametrize("variant", [4, 6])
+@pytest.mark.parametrize("esz", [4, 16])
+def test_any_pairing(ctx, oracle, esz, variant):
+    """RZ_ANY_PAIRING (early pairing, paper p.14): a slice pairs its own
+    orphans first and only the rest go to the global exchange.  The array may
+    differ from the canonical one, so the check is the Redzone contract
+    (length, survivors with their payload bits, every kept item outside the
+    tail in place), and the exchange sees fewer pairs than the canonical
+    pairing has."""
+    for n, frac, ms, pattern in [((1 << 18) + 5, 0.3, 64, "uniform"), (1 << 20, 0.05, 512, "tail_heavy"),
+                                 (70000, 0.5, 7, "clustered"), (1 << 16, 0.9, 4096, "uniform"),
+                                 (3000, 0.4, 1, "uniform")]:
+        k = int(n * frac)
+        A = _payload(esz, n, seed=70)
+        R = W.removals(pattern, n, k, seed=71)
+        Rn = R.numpy()
+        sz = n - k
+        dead = np.zeros(k, bool)
+        dead[Rn[Rn >= sz] - sz] = True
+        canonical_pairs = int(((Rn < sz) & dead).sum())
+        announce("any pairing", esz, variant, n, k, pattern)
+        dA, dR = A.cuda(), R.cuda()
+        cfg = rz.TileConfig(256, 4, ms, variant, 0, 0)
+        assert ctx.remove(dA, dR, config=cfg, flags=rz.RZ_ANY_PAIRING | rz.RZ_STATS) == n - k
+        s = ctx.last_stats()
+        got = dA.cpu().numpy()
+        bad, fb = oracle.check(A.numpy(), got, Rn, mode=0)
+        assert bad == 0, (n, k, pattern, bad, fb)
+        assert s.error_flags == 0
+        assert s.orphan_pairs <= canonical_pairs
+        if ms > 1 and canonical_pairs > 1000:
+            assert s.orphan_pairs < canonical_pairs  # slices with both kinds paired some themselves
