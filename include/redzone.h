@@ -104,8 +104,10 @@ rz_status rz_add_remove(rz_ctx* ctx, void* d_data, int64_t n, int64_t capacity, 
                         const void* d_add, int64_t g, int32_t* d_removals, int64_t k, uint32_t flags, void* stream);
 /* In-place gather by a preserve list (paper p.14, eq. 3): A'[i] = A[P[i]] for
  * i < p, with P strictly ascending and P[p-1] < n (checked with RZ_VALIDATE).
- * The survivors keep their order.  Gathers into context scratch (p items),
- * then copies back; it pays when p is small (when most items go). */
+ * The survivors keep their order.  From p = 2^22 on, one in-place pass (tiles
+ * write in launch order once all earlier tiles have read their sources; no
+ * scratch for the items); below, a gather into context scratch (p items) and
+ * a copy back. */
 rz_status rz_gather_preserve(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, const int32_t* d_keep,
                              int64_t p, uint32_t flags, void* stream);
 /* The mark step alone, into a caller-owned tail bitmap (zero beforehand),

@@ -12,7 +12,7 @@ namespace rz {
 // Words of the control block (uint32 index).  Each group sits on its own
 // 128-byte line so that polling one does not contend with updates of another.
 enum CtrlWord : int {
-  kCtrlEpoch = 0,         // look-back epoch of the O(n) partition (bumped by its mark kernel)
+  kCtrlEpoch = 0,         // flag epoch of the O(n) partition and the one-pass gather (bumped per call)
   kCtrlError = 32,        // device-detected contract violations (bit flags)
   kCtrlBarCount = 64,     // uint64 arrival counter of the in-kernel grid barrier
   kCtrlBarEpoch = 96,     // launch epoch of the in-kernel grid barrier

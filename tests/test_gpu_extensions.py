@@ -80,6 +80,32 @@ def test_gather_preserve(ctx, esz):
             assert got.shape == A_np.shape
 
 
+@pytest.mark.parametrize("esz", [4, 8, 16])
+def test_gather_preserve_one_pass(ctx, esz):
+    """From p = 2^22 on the gather is one in-place pass ordered by tile flags;
+    calls of different p (so different tile counts) follow each other, with a
+    small two-pass call between them."""
+    n = (1 << 23) + 5
+    A = W.payload(esz, n, seed=58)
+    A_np = A.numpy()
+    rng = np.random.default_rng(59)
+    R = W.clustered_removals(n, n // 3, seed=60).numpy()
+    keep = np.ones(n, bool)
+    keep[R] = False
+    cases = [("uniform 0.99", np.sort(rng.choice(n, int(n * 0.99), replace=False)).astype(np.int32)),
+             ("uniform 0.5", np.sort(rng.choice(n, n // 2, replace=False)).astype(np.int32)),
+             ("small, two passes", np.sort(rng.choice(n, 5000, replace=False)).astype(np.int32)),
+             ("clustered complement", np.nonzero(keep)[0].astype(np.int32)),
+             ("identity", np.arange(n, dtype=np.int32))]
+    for name, P in cases:
+        announce("gather one pass", esz, n, name, len(P))
+        dA = A.cuda()
+        assert ctx.gather_preserve(dA, torch.from_numpy(P).cuda(), n=n) == len(P)
+        torch.cuda.synchronize()
+        mm = first_mismatch(dA.cpu().numpy()[: len(P)], A_np[P])
+        assert mm < 0, (name, mm)
+
+
 def test_gather_preserve_equals_stable_removal(ctx, oracle):
     """With P the ascending complement of R, the gather is the stable O(n)
     removal: the oracle's restatement of std::remove on the marked copy."""
