@@ -9,12 +9,20 @@
 //   MARK  j       marks of removals [jL, (j+1)L) into the list's tail bitmap;
 //   MOVE  j       slice j, after every MARK of the list has finished;
 //   PAIR  j       share j of the orphan pairs, after every MOVE has finished.
-// Items are dequeued in index order with an atomic ticket, and every wait is on
-// items with smaller indices (all MARK items precede all MOVE items, which
-// precede all PAIR items), so the queue drains whatever the number of
-// resident workgroups: no cooperative launch is needed.  The last workgroup to
-// leave resets the ticket and the per-list counters, so the next launch (or
-// graph replay) starts clean without a memset.
+// A batch is planned into one queue per XCD: lists go to XCDs by longest-
+// processing-time order on their removal counts, and all items of a list sit
+// in its XCD's queue.  Workgroup w drains queue w mod 8 first (the XCD the
+// hardware dispatches it to), then helps with the others.  The holes of a list
+// are then written from one XCD, where writes to the same line meet in that
+// XCD's L2 instead of each costing a DRAM read-modify-write.
+// Within a queue the lists follow each other (MARK items of a list, then its
+// MOVE items, then its PAIR items; one-slice lists last), so each XCD writes
+// into few lists at a time.  Items of a queue are dequeued in index order with
+// an atomic ticket, and every wait is on items of the same list, which sit
+// earlier in the same queue, so the queues drain whatever the number of
+// resident workgroups: no cooperative launch is needed.
+// The last workgroup to leave resets the tickets and the per-list counters, so
+// the next launch (or graph replay) starts clean without a memset.
 #include <string.h>
 
 #include <algorithm>
@@ -40,8 +48,14 @@ struct ListPlan {
 };
 static_assert(sizeof(ListPlan) == 64, "ListPlan is one 64-byte record");
 
-// queue control words (uint32 index into the queue control block)
-enum QueueWord : int { kQTicket = 0, kQExit = 32, kQWords = 64 };
+// queue control words (uint32 index into the queue control block): the ticket
+// of queue x at kQTicket + 32 x (a line each), then the exit count
+constexpr int kQueues = 8;
+enum QueueWord : int { kQTicket = 0, kQExit = 32 * kQueues, kQWords = 32 * kQueues + 32 };
+// queue x holds items [off[x], off[x + 1])
+struct QueueSplit {
+  uint32_t off[kQueues + 1];
+};
 
 template <typename T, int WG, int IPT>
 __device__ __forceinline__ void run_item(const ListPlan& P, uint32_t li, uint32_t kind, uint32_t j,
@@ -79,7 +93,7 @@ __device__ __forceinline__ void run_item(const ListPlan& P, uint32_t li, uint32_
 // (q < 2m) or PAIR q-2m; with m == 1 the only item is SINGLE.
 template <int WG, int IPT>
 __global__ __launch_bounds__(WG) void rz_queue(ListPlan single, const ListPlan* __restrict__ plans,
-                                                const uint2* __restrict__ items, uint32_t nitems, uint32_t chunk,
+                                                const uint2* __restrict__ items, QueueSplit qs, uint32_t chunk,
                                                 uint32_t* __restrict__ qctl, uint32_t* __restrict__ done,
                                                 uint32_t nlists, uint32_t* __restrict__ bits,
                                                 uint64_t* __restrict__ counts, uint32_t* __restrict__ ctrl) {
@@ -87,42 +101,53 @@ __global__ __launch_bounds__(WG) void rz_queue(ListPlan single, const ListPlan* 
   __shared__ PairShared<WG, kQueueMaxSlices> psh;
   __shared__ uint32_t s_q, s_last;
   const uint32_t tid = threadIdx.x;
-  for (;;) {
-    if (tid == 0) s_q = add_relaxed(&qctl[kQTicket], chunk);
-    __syncthreads();
-    const uint32_t q0 = s_q;
-    __syncthreads();
-    if (q0 >= nitems) break;
-    const uint32_t q1 = min(q0 + chunk, nitems);
-    for (uint32_t q = q0; q < q1; ++q) {
-      uint32_t li, kind, j;
-      ListPlan P;
-      if (items) {
-        const uint2 it = items[q];
-        li = it.x & 0x0fffffffu;
-        kind = it.x >> 28;
-        j = it.y;
-        P = plans[li];
-      } else {
-        P = single;
-        li = 0;
-        if (P.m == 1) {
-          kind = kSingle;
-          j = 0;
-        } else if (q < P.m) {
-          kind = kMark;
-          j = q;
-        } else if (q < 2 * P.m) {
-          kind = kMoveItem;
-          j = q - P.m;
+  const uint32_t home = blockIdx.x & (kQueues - 1);
+  for (uint32_t r = 0; r < (uint32_t)kQueues; ++r) {
+    const uint32_t x = (home + r) & (kQueues - 1);
+    uint32_t lo = 0, hi = 0;  // selects, not an indexed copy of the argument
+#pragma unroll
+    for (int i = 0; i < kQueues; ++i) {
+      if ((uint32_t)i == x) lo = qs.off[i];
+      if ((uint32_t)i == x) hi = qs.off[i + 1];
+    }
+    if (lo == hi) continue;
+    for (;;) {
+      if (tid == 0) s_q = lo + add_relaxed(&qctl[kQTicket + 32 * x], chunk);
+      __syncthreads();
+      const uint32_t q0 = s_q;
+      __syncthreads();
+      if (q0 >= hi) break;
+      const uint32_t q1 = min(q0 + chunk, hi);
+      for (uint32_t q = q0; q < q1; ++q) {
+        uint32_t li, kind, j;
+        ListPlan P;
+        if (items) {
+          const uint2 it = items[q];
+          li = it.x & 0x0fffffffu;
+          kind = it.x >> 28;
+          j = it.y;
+          P = plans[li];
         } else {
-          kind = kPair;
-          j = q - 2 * P.m;
+          P = single;
+          li = 0;
+          if (P.m == 1) {
+            kind = kSingle;
+            j = 0;
+          } else if (q < P.m) {
+            kind = kMark;
+            j = q;
+          } else if (q < 2 * P.m) {
+            kind = kMoveItem;
+            j = q - P.m;
+          } else {
+            kind = kPair;
+            j = q - 2 * P.m;
+          }
         }
+        if (P.esz == 4) run_item<uint32_t, WG, IPT>(P, li, kind, j, done, bits, counts, ctrl, msh, psh);
+        else if (P.esz == 8) run_item<uint64_t, WG, IPT>(P, li, kind, j, done, bits, counts, ctrl, msh, psh);
+        else run_item<uint4, WG, IPT>(P, li, kind, j, done, bits, counts, ctrl, msh, psh);
       }
-      if (P.esz == 4) run_item<uint32_t, WG, IPT>(P, li, kind, j, done, bits, counts, ctrl, msh, psh);
-      else if (P.esz == 8) run_item<uint64_t, WG, IPT>(P, li, kind, j, done, bits, counts, ctrl, msh, psh);
-      else run_item<uint4, WG, IPT>(P, li, kind, j, done, bits, counts, ctrl, msh, psh);
     }
   }
   // the last workgroup out resets the queue state for the next launch
@@ -134,7 +159,7 @@ __global__ __launch_bounds__(WG) void rz_queue(ListPlan single, const ListPlan* 
   if (s_last) {
     for (uint32_t i = tid; i < 2 * nlists; i += WG) st_relaxed(&done[i], 0u);
     if (tid == 0) {
-      st_relaxed(&qctl[kQTicket], 0u);
+      for (int x = 0; x < kQueues; ++x) st_relaxed(&qctl[kQTicket + 32 * x], 0u);
       st_relaxed(&qctl[kQExit], 0u);
     }
   }
@@ -143,8 +168,16 @@ __global__ __launch_bounds__(WG) void rz_queue(ListPlan single, const ListPlan* 
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-typedef void (*QueueKernel)(ListPlan, const ListPlan*, const uint2*, uint32_t, uint32_t, uint32_t*, uint32_t*,
+typedef void (*QueueKernel)(ListPlan, const ListPlan*, const uint2*, QueueSplit, uint32_t, uint32_t*, uint32_t*,
                             uint32_t, uint32_t*, uint64_t*, uint32_t*);
+
+// every item in queue 0 (one list, or a plan that is not split)
+static QueueSplit one_queue(uint32_t nitems) {
+  QueueSplit q;
+  q.off[0] = 0;
+  for (int x = 1; x <= kQueues; ++x) q.off[x] = nitems;
+  return q;
+}
 
 static QueueKernel queue_kernel(int wg, int ipt) {
   if (wg == 256 && ipt == 4) return rz_queue<256, 4>;
@@ -213,7 +246,7 @@ rz_status launch_queue_single(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, 
   if ((st = prof_mark(ctx, flags, 0, s)) != RZ_OK) return st;
   if ((st = prof_mark(ctx, flags, 1, s)) != RZ_OK) return st;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(cfg.wg_size), 0, s, P, (const ListPlan*)nullptr, (const uint2*)nullptr,
-                     nitems, 1u, qctl, qctl + kQWords, 1u, (uint32_t*)ctx->tail_bits.ptr,
+                     one_queue(nitems), 1u, qctl, qctl + kQWords, 1u, (uint32_t*)ctx->tail_bits.ptr,
                      (uint64_t*)ctx->counts.ptr, (uint32_t*)ctx->ctrl.ptr);
   RZ_HIP(hipGetLastError());
   if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;
@@ -232,6 +265,7 @@ struct rz_batch {
   uint32_t flags = 0;
   uint32_t nitems = 0, grid = 0, chunk = 1;
   rz::QueueKernel kern = nullptr;
+  rz::QueueSplit split;
   rz::DeviceBuffer plans, items, qctl, bits, counts, orphans;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -241,6 +275,7 @@ struct rz_batch {
   std::vector<rz::ListPlan> h_plans;
   std::vector<uint2> h_items;
   std::vector<uint64_t> h_keys;
+  std::vector<uint8_t> h_xcd;
   void* staging = nullptr;
   size_t staging_bytes = 0;
   hipEvent_t staged_ev = nullptr;
@@ -290,26 +325,37 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
   }
   if (bits_words >= (1ull << 32) || count_words >= (1ull << 32)) return RZ_EINVAL;
   std::sort(keys.begin(), keys.end());
+  // lists to XCD queues, largest first onto the least loaded queue
+  std::vector<uint8_t>& xcd = b->h_xcd;
+  xcd.assign((size_t)nlists, 0);
+  uint64_t load[kQueues] = {};
+  for (uint64_t key : keys) {
+    const uint32_t li = (uint32_t)key;
+    int best = 0;
+    for (int x = 1; x < kQueues; ++x)
+      if (load[x] < load[best]) best = x;
+    xcd[li] = (uint8_t)best;
+    load[best] += (uint64_t)plans[li].k + 256u;  // removals, plus a fixed cost per list
+  }
   auto push = [&](uint32_t li, uint32_t kind, uint32_t j) { items.push_back(make_uint2(li | (kind << 28), j)); };
-  for (uint64_t key : keys) {
-    const uint32_t li = (uint32_t)key;
-    if (plans[li].m > 1)
+  QueueSplit split;
+  for (int x = 0; x < kQueues; ++x) {
+    split.off[x] = (uint32_t)items.size();
+    // list after list, largest first: an XCD works on few lists at a time,
+    // so the lines of a list's holes stay in its L2 while they are written
+    for (uint64_t key : keys) {
+      const uint32_t li = (uint32_t)key;
+      if (xcd[li] != x || plans[li].m <= 1) continue;
       for (uint32_t j = 0; j < plans[li].m; ++j) push(li, kMark, j);
-  }
-  for (uint64_t key : keys) {
-    const uint32_t li = (uint32_t)key;
-    if (plans[li].m == 1) push(li, kSingle, 0);
-  }
-  for (uint64_t key : keys) {
-    const uint32_t li = (uint32_t)key;
-    if (plans[li].m > 1)
       for (uint32_t j = 0; j < plans[li].m; ++j) push(li, kMoveItem, j);
-  }
-  for (uint64_t key : keys) {
-    const uint32_t li = (uint32_t)key;
-    if (plans[li].m > 1)
       for (uint32_t j = 0; j < plans[li].npair; ++j) push(li, kPair, j);
+    }
+    for (uint64_t key : keys) {
+      const uint32_t li = (uint32_t)key;
+      if (xcd[li] == x && plans[li].m == 1) push(li, kSingle, 0);
+    }
   }
+  split.off[kQueues] = (uint32_t)items.size();
   rz_status st;
   const size_t plan_bytes = plans.size() * sizeof(ListPlan), item_bytes = items.size() * sizeof(uint2);
   if ((st = ensure(ctx, b->plans, std::max<size_t>(64, plan_bytes), s)) != RZ_OK ||
@@ -354,6 +400,7 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
   b->cfg = cfg;
   b->flags = flags;
   b->nitems = (uint32_t)items.size();
+  b->split = split;
   b->chunk = cfg.chunk > 0 ? (uint32_t)cfg.chunk : 1u;
   b->grid = std::max<uint32_t>(1, std::min<uint32_t>(b->grid_cap, b->nitems));
   b->info.nitems = b->nitems;
@@ -409,7 +456,7 @@ rz_status rz_batch_launch(rz_batch* b, void* stream) {
   }
   uint32_t* q = (uint32_t*)b->qctl.ptr;
   hipLaunchKernelGGL(b->kern, dim3(b->grid), dim3(b->cfg.wg_size), 0, s, ListPlan{},
-                     (const ListPlan*)b->plans.ptr, (const uint2*)b->items.ptr, b->nitems, b->chunk, q,
+                     (const ListPlan*)b->plans.ptr, (const uint2*)b->items.ptr, b->split, b->chunk, q,
                      q + kQWords, (uint32_t)b->nlists, (uint32_t*)b->bits.ptr, (uint64_t*)b->counts.ptr,
                      (uint32_t*)b->ctx->ctrl.ptr);
   RZ_HIP(hipGetLastError());
@@ -429,7 +476,7 @@ rz_status rz_batch_capture(rz_batch* b, void* stream) {
   }
   RZ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   hipLaunchKernelGGL(b->kern, dim3(b->grid), dim3(b->cfg.wg_size), 0, s, ListPlan{},
-                     (const ListPlan*)b->plans.ptr, (const uint2*)b->items.ptr, b->nitems, b->chunk,
+                     (const ListPlan*)b->plans.ptr, (const uint2*)b->items.ptr, b->split, b->chunk,
                      (uint32_t*)b->qctl.ptr, (uint32_t*)b->qctl.ptr + kQWords, (uint32_t)b->nlists,
                      (uint32_t*)b->bits.ptr, (uint64_t*)b->counts.ptr, (uint32_t*)b->ctx->ctrl.ptr);
   hipGraph_t g = nullptr;
