@@ -174,8 +174,9 @@ rz_status rz_partition_on(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_by
 /* ---- batched Redzone ----------------------------------------------------- */
 /* No counterpart in the reference (it removes from one list per launch).
  * Many independent lists are compacted by one persistent kernel that pulls
- * (list, slice) work items from a device queue; every list gets exactly the
- * result rz_remove would give it.  Lists must not overlap. */
+ * (list, slice) work items from device queues, one per XCD (each list's items
+ * in one queue, list after list); every list gets exactly the result
+ * rz_remove would give it.  Lists must not overlap. */
 typedef struct rz_list_desc {
   void* data;          /* device: n items of elem_bytes each */
   int32_t* removals;   /* device: k distinct indices in [0, n) */
@@ -188,7 +189,7 @@ typedef struct rz_list_desc {
 typedef struct rz_batch rz_batch;
 
 typedef struct rz_batch_info {
-  int64_t nitems;        /* work items in the queue */
+  int64_t nitems;        /* work items in the queues */
   int64_t bitmap_bytes;  /* tail bitmap scratch */
   int32_t grid;          /* persistent workgroups launched */
   int32_t lists_active;  /* lists with 0 < k < n */
