@@ -118,6 +118,22 @@ def test_batch_config4_shape(ctx, oracle):
     b.close()
 
 
+def test_batch_fewer_lists_than_queues(ctx, oracle):
+    """One queue per XCD: with fewer multi-slice lists than queues most
+    workgroups find their own queue empty and take another queue's items."""
+    for specs in ([(1 << 20, 1 << 17, 4, "uniform")],
+                  [(1 << 20, 1 << 17, 4, "uniform"), (1 << 19, 5000, 8, "clustered"),
+                   (300000, 250000, 16, "tail_heavy"), (5000, 0, 4, "uniform")]):
+        L = Lists(specs, seed=4000 + len(specs))
+        for cfg in [None, rz.TileConfig(256, 4, 0, 2, 1, 1)]:
+            announce("batch few lists", len(specs), cfg)
+            L.restore()
+            b = ctx.batch(L.pairs(), config=cfg)
+            b.launch()
+            L.check(oracle, f"few lists {len(specs)} {cfg}")
+            b.close()
+
+
 def test_batch_graph_replay(ctx, oracle):
     """Capture once, replay several times with eager restores and syncs in between;
     every replay must equal the oracle."""
