@@ -16,8 +16,9 @@
 // are then written from one XCD, where writes to the same line meet in that
 // XCD's L2 instead of each costing a DRAM read-modify-write.
 // Within a queue the lists follow each other (MARK items of a list, then its
-// MOVE items, then its PAIR items; one-slice lists last), so each XCD writes
-// into few lists at a time.  Items of a queue are dequeued in index order with
+// MOVE items, then its PAIR items, with the next list's MARK items pulled
+// ahead of the MOVE items and the PAIR items behind the next list's MOVE
+// items; one-slice lists last), so each XCD writes into few lists at a time.  Items of a queue are dequeued in index order with
 // an atomic ticket, and every wait is on items of the same list, which sit
 // earlier in the same queue, so the queues drain whatever the number of
 // resident workgroups: no cooperative launch is needed.
@@ -276,6 +277,7 @@ struct rz_batch {
   std::vector<uint2> h_items;
   std::vector<uint64_t> h_keys;
   std::vector<uint8_t> h_xcd;
+  std::vector<uint32_t> h_seq;
   void* staging = nullptr;
   size_t staging_bytes = 0;
   hipEvent_t staged_ev = nullptr;
@@ -342,14 +344,26 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
   for (int x = 0; x < kQueues; ++x) {
     split.off[x] = (uint32_t)items.size();
     // list after list, largest first: an XCD works on few lists at a time,
-    // so the lines of a list's holes stay in its L2 while they are written
+    // so the lines of a list's holes stay in its L2 while they are written.
+    // The next list's MARK items go ahead of this list's MOVE items, and a
+    // list's PAIR items after the next list's MOVE items, so that a MOVE or
+    // PAIR item rarely waits for the items it depends on.
+    std::vector<uint32_t>& seq = b->h_seq;
+    seq.clear();
     for (uint64_t key : keys) {
       const uint32_t li = (uint32_t)key;
-      if (xcd[li] != x || plans[li].m <= 1) continue;
-      for (uint32_t j = 0; j < plans[li].m; ++j) push(li, kMark, j);
-      for (uint32_t j = 0; j < plans[li].m; ++j) push(li, kMoveItem, j);
-      for (uint32_t j = 0; j < plans[li].npair; ++j) push(li, kPair, j);
+      if (xcd[li] == x && plans[li].m > 1) seq.push_back(li);
     }
+    auto push_all = [&](uint32_t li, uint32_t kind, uint32_t cnt) {
+      for (uint32_t j = 0; j < cnt; ++j) push(li, kind, j);
+    };
+    for (size_t i = 0; i < seq.size(); ++i) {
+      if (i == 0) push_all(seq[0], kMark, plans[seq[0]].m);
+      if (i + 1 < seq.size()) push_all(seq[i + 1], kMark, plans[seq[i + 1]].m);
+      push_all(seq[i], kMoveItem, plans[seq[i]].m);
+      if (i >= 1) push_all(seq[i - 1], kPair, plans[seq[i - 1]].npair);
+    }
+    if (!seq.empty()) push_all(seq.back(), kPair, plans[seq.back()].npair);
     for (uint64_t key : keys) {
       const uint32_t li = (uint32_t)key;
       if (xcd[li] == x && plans[li].m == 1) push(li, kSingle, 0);
