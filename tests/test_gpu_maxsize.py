@@ -1,8 +1,7 @@
 """The largest list the boundary accepts, n = 2^31 - 1 (the reference's sizes
-are `int`, redzone.hxx:627): the fused Redzone forms and the O(n) partition
-array-exact against the oracle, and the one-pass preserve-list gather against
-torch indexing on the device.  The dense form is not run at this size."""
-import numpy as np
+are `int`, redzone.hxx:627): the fused Redzone forms, the dense form and the
+O(n) partition array-exact against the oracle, and the one-pass preserve-list
+gather against torch indexing on the device."""
 import pytest
 import torch
 
@@ -36,7 +35,8 @@ def _removals(ctx, k, seed):
 
 
 @pytest.mark.parametrize("k,config", [(1 << 20, None),                               # tuned: mark + fused move/pair
-                                      (1 << 14, rz.TileConfig(256, 4, 512, 6, 12, 1))])  # one launch
+                                      (1 << 14, rz.TileConfig(256, 4, 512, 6, 12, 1)),   # one launch
+                                      (1 << 29, None)])  # tuned: dense form, bucketed marks at their 2^29 limit
 def test_max_size_redzone(ctx, oracle, big, k, config):
     A0, host = big
     announce("max size redzone", N_MAX, k, config)
