@@ -80,3 +80,24 @@ def test_max_size_gather(ctx, big):
     torch.cuda.synchronize()
     assert torch.equal(A[:p], A0[P.long()])
     assert torch.equal(A[p:], A0[p:]), "the gather wrote past p"
+
+
+def test_max_size_rec16(ctx, oracle, big):
+    """16-byte records at n = 2^31 - 1 (32 GiB, byte offsets past 2^32)."""
+    k = 1 << 20
+    announce("max size rec16", N_MAX, k)
+    A = torch.empty((N_MAX, 4), dtype=torch.int32, device="cuda")
+    ctx.gen_payload(A, 161)
+    R = _removals(ctx, k, 162)
+    torch.cuda.synchronize()
+    host = A.cpu().numpy()
+    R_np = R.cpu().numpy()
+    assert ctx.remove(A, R) == N_MAX - k
+    torch.cuda.synchronize()
+    got = A.cpu().numpy()
+    del A
+    torch.cuda.empty_cache()
+    want = expected_canonical(oracle, host, R_np)
+    del host
+    m = first_mismatch(got[: N_MAX - k], want[: N_MAX - k])
+    assert m < 0, f"first mismatch at {m}"
