@@ -13,8 +13,10 @@
 // processing-time order on their removal counts, and all items of a list sit
 // in its XCD's queue.  Workgroup w drains queue w mod 8 first (the XCD the
 // hardware dispatches it to), then helps with the others.  The holes of a list
-// are then written from one XCD, where writes to the same line meet in that
-// XCD's L2 instead of each costing a DRAM read-modify-write.
+// are then written from one XCD, which works on one or two lists at a time.
+// On MI355X that made the 4096-list batch 1.7x faster than one shared queue,
+// although the L2s still write back one 32-byte sector per hole: the saving is
+// on the memory side, where the writes now arrive local in time and address.
 // Within a queue the lists follow each other (MARK items of a list, then its
 // MOVE items, then its PAIR items, with the next list's MARK items pulled
 // ahead of the MOVE items and the PAIR items behind the next list's MOVE
@@ -344,7 +346,7 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
   for (int x = 0; x < kQueues; ++x) {
     split.off[x] = (uint32_t)items.size();
     // list after list, largest first: an XCD works on few lists at a time,
-    // so the lines of a list's holes stay in its L2 while they are written.
+    // so its writes stay within one or two lists.
     // The next list's MARK items go ahead of this list's MOVE items, and a
     // list's PAIR items after the next list's MOVE items, so that a MOVE or
     // PAIR item rarely waits for the items it depends on.
