@@ -12,7 +12,8 @@ namespace rz {
 // Words of the control block (uint32 index).  Each group sits on its own
 // 128-byte line so that polling one does not contend with updates of another.
 enum CtrlWord : int {
-  kCtrlEpoch = 0,         // flag epoch of the O(n) partition and the one-pass gather (bumped per call)
+  kCtrlEpoch = 0,         // flag epoch of the O(n) partition (bumped by its mark kernel) and of the
+                          // one-pass gather (advanced by its last tile)
   kCtrlError = 32,        // device-detected contract violations (bit flags)
   kCtrlBarCount = 64,     // uint64 arrival counter of the in-kernel grid barrier
   kCtrlBarEpoch = 96,     // launch epoch of the in-kernel grid barrier
@@ -23,7 +24,8 @@ enum CtrlWord : int {
   kCtrlMoveExit = 240,    // workgroups of the fused kernel that have finished (reset by the last)
   kCtrlMarkDone = 256,    // mark workgroups of the one-launch form that are done (reset by the last of them)
   kCtrlMarkReady = 288,   // 8 copies, one line apart (one per XCD): move-pair epoch + 1 once every mark is in
-  kCtrlWords = 544
+  kCtrlGatherExit = 544,  // tiles of the one-pass gather that have finished (reset by the last)
+  kCtrlWords = 576
 };
 
 enum ErrorBits : uint32_t {

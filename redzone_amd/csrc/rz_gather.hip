@@ -41,7 +41,10 @@ __global__ __launch_bounds__(WG) void rz_gather_kernel(const T* __restrict__ A, 
 // its group have loaded), waits until every earlier tile has loaded, and
 // writes its outputs.  The sources of a tile lie at or after its first output
 // (P ascends, P[i] >= i), so only earlier tiles' sources can sit in its output
-// range, and later tiles never write below it.  Flags carry the call's epoch.
+// range, and later tiles never write below it.  Flags carry the call's tag,
+// the epoch word plus one; the last tile to leave (every tile has read the
+// epoch by then) stores the tag as the new epoch, so no flag is ever cleared
+// and no separate launch advances the epoch.
 template <typename T, int WG, int IPT>
 __global__ __launch_bounds__(WG) void rz_gather_inplace(T* __restrict__ A, uint32_t n,
                                                         const int32_t* __restrict__ P, uint32_t p,
@@ -50,7 +53,8 @@ __global__ __launch_bounds__(WG) void rz_gather_inplace(T* __restrict__ A, uint3
                                                         uint32_t* __restrict__ ctrl) {
   constexpr uint32_t TILE = WG * IPT;
   const uint32_t t = blockIdx.x, i0 = t * TILE;
-  const uint32_t epoch = ld_relaxed(&ctrl[kCtrlEpoch]) & 0x3fffffffu;
+  const uint32_t e0 = ld_relaxed(&ctrl[kCtrlEpoch]);
+  const uint32_t epoch = (e0 + 1u) & 0x3fffffffu;  // this call's tag
   uint32_t src[IPT];
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
@@ -106,9 +110,11 @@ __global__ __launch_bounds__(WG) void rz_gather_inplace(T* __restrict__ A, uint3
     const uint32_t i = i0 + j * WG + threadIdx.x;
     if (i < p) A[i] = v[j];
   }
+  if (threadIdx.x == 0 && add_relaxed(&ctrl[kCtrlGatherExit], 1u) == ntiles - 1u) {
+    st_relaxed(&ctrl[kCtrlGatherExit], 0u);
+    st_relaxed(&ctrl[kCtrlEpoch], e0 + 1u);
+  }
 }
-
-__global__ void rz_gather_epoch(uint32_t* __restrict__ ctrl) { add_relaxed(&ctrl[kCtrlEpoch], 1u); }
 
 template <typename T, int IPT>
 static rz_status gather_inplace_launch(rz_ctx* ctx, T* A, uint32_t n, const int32_t* P, uint32_t p, hipStream_t s) {
@@ -120,8 +126,6 @@ static rz_status gather_inplace_launch(rz_ctx* ctx, T* A, uint32_t n, const int3
   if ((st = ensure(ctx, ctx->pgcnt, (size_t)groups * 4, s)) != RZ_OK) return st;
   if ((st = ensure(ctx, ctx->pgdone, (size_t)groups * 4, s)) != RZ_OK) return st;
   uint32_t* ctrl = (uint32_t*)ctx->ctrl.ptr;
-  // a fresh epoch for the flags (the partition's flags share the counter and the buffers)
-  hipLaunchKernelGGL(rz_gather_epoch, dim3(1), dim3(1), 0, s, ctrl);
   hipLaunchKernelGGL((rz_gather_inplace<T, WG, IPT>), dim3(tiles), dim3(WG), 0, s, A, n, P, p,
                      (uint32_t*)ctx->ploaded.ptr, (uint32_t*)ctx->pgcnt.ptr, (uint32_t*)ctx->pgdone.ptr, tiles, ctrl);
   RZ_HIP(hipGetLastError());
