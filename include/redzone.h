@@ -42,14 +42,18 @@ typedef enum rz_status {
   RZ_ECOOP = 4,    /* cooperative launch refused (grid not co-resident) */
   RZ_EHIP = 5,     /* a HIP runtime call failed */
   RZ_ENOMEM = 6,   /* scratch allocation failed */
-  RZ_ENOTUNED = 7  /* no learned model loaded where one is required */
+  RZ_ENOTUNED = 7, /* no learned model loaded where one is required */
+  RZ_EDEVICE = 8   /* a check made on the device failed (see rz_check_errors): the output of the calls since the
+                      last check is unspecified; the context's scratch has been reset */
 } rz_status;
 
 /* flags */
 #define RZ_PRESERVE_REMOVALS 0x1u /* leave d_removals unchanged (dest orphans go to context scratch) */
-#define RZ_VALIDATE 0x2u          /* check range and duplicates first; synchronises the stream */
+#define RZ_VALIDATE 0x2u          /* check range and duplicates first, and the device error word after the
+                                     call (as rz_check_errors); synchronises the stream */
 #define RZ_COOPERATIVE 0x4u       /* single cooperative launch with in-kernel grid barriers */
-#define RZ_STATS 0x8u             /* record per-call statistics readable with rz_last_stats (synchronises) */
+#define RZ_STATS 0x8u             /* record per-call statistics readable with rz_last_stats, and check the
+                                     device error word after the call (as rz_check_errors); synchronises */
 #define RZ_PROFILE 0x10u          /* time each kernel with events on the stream; read with rz_last_profile */
 #define RZ_SORTED 0x20u           /* d_removals is strictly ascending: the mark step reads only its red-zone suffix
                                      (checked by RZ_VALIDATE; if it is not true the result is unspecified) */
@@ -122,9 +126,19 @@ typedef struct rz_stats {
   int64_t tail_removals;  /* k_z = |{r in R : r >= n-k}| */
   int64_t orphan_pairs;   /* dest orphans (== source orphans) */
   int64_t slices;
-  int32_t error_flags;    /* nonzero: input violated the contract (seen on device) */
+  int32_t error_flags;    /* the device error word read by that call (RZ_ERR_* bits; 0 = none) */
   int32_t pad;
 } rz_stats;
+
+/* Bits of the device error word.  The kernels OR them in; they stay set until
+ * a checking call (rz_check_errors, or a call with RZ_STATS or RZ_VALIDATE)
+ * reads and clears the word. */
+#define RZ_ERR_ORPHAN_MISMATCH 0x1u /* dest and source orphan totals differ: the removal list broke the contract
+                                       (a duplicate or an index out of range); the reference asserts the same
+                                       equality in debug builds, redzone.hxx:905 */
+#define RZ_ERR_WAIT_TIMEOUT 0x2u    /* an in-kernel wait on other workgroups (grid barrier, mark count, ready
+                                       word) gave up: workgroups were not dispatched as the launch assumes */
+#define RZ_ERR_LOOKBACK_TIMEOUT 0x4u /* a decoupled look-back gave up, for the same reason */
 
 /* ---- context ------------------------------------------------------------ */
 /* The context owns all device scratch (tail bitmap, slice counts, look-back
@@ -135,6 +149,23 @@ typedef struct rz_stats {
 rz_status rz_ctx_create(rz_ctx** out, int32_t device);
 rz_status rz_ctx_destroy(rz_ctx* ctx);
 const char* rz_status_string(rz_status s);
+/* Waits for `stream`, then reads and clears the device error word of the
+ * context (the checks the kernels make: orphan totals, and every bounded wait
+ * on other workgroups; the reference's counterpart is the debug assert of
+ * redzone.hxx:905, which stops the program).  *error_flags (may be null) gets
+ * the RZ_ERR_* bits.  Returns RZ_OK when no bit was set.  Otherwise every call
+ * on the context since the previous check may have produced a wrong array: the
+ * scratch the kernels keep zero between calls is zeroed again (this
+ * synchronises the device) and RZ_EDEVICE is returned.  Errors of calls on
+ * other streams are seen once those calls have finished. */
+rz_status rz_check_errors(rz_ctx* ctx, void* stream, uint32_t* error_flags);
+/* Test hook for the failure path: the bounded in-kernel waits give up after
+ * `spin_cap` polls (0 = their own limits), and with stall != 0 the waits of
+ * the Redzone forms (look-back, ready word, counters) never see their
+ * condition met, so each of them gives up and sets its RZ_ERR_* bit; every
+ * kernel still runs to its end and stays inside its buffers.  Synchronous.
+ * No counterpart in the reference. */
+rz_status rz_debug_set_waits(rz_ctx* ctx, uint32_t spin_cap, uint32_t stall);
 /* Device properties seen by the context: compute units and HBM peak (GB/s) used for reports. */
 rz_status rz_device_info(rz_ctx* ctx, int32_t* cu_count, char* arch, int32_t arch_len);
 
@@ -202,7 +233,8 @@ typedef struct rz_batch_info {
 rz_status rz_batch_create(rz_ctx* ctx, const rz_list_desc* lists, int32_t nlists, uint32_t flags,
                           const rz_tile_config* cfg, rz_batch** out);
 /* One persistent launch (or a replay of the captured graph); no allocation,
- * no memset, so it may itself be captured by the caller. */
+ * no memset, so it may itself be captured by the caller.  RZ_EINVAL once the
+ * batch's context has been destroyed (the batch can still be destroyed). */
 rz_status rz_batch_launch(rz_batch* b, void* stream);
 /* Captures the launch into a hipGraph owned by the batch; later
  * rz_batch_launch calls replay it. */
@@ -235,11 +267,14 @@ rz_status rz_tune_candidates(int32_t batch, rz_tile_config* out, int32_t max, in
 /* Times every candidate (null = built-in list) on a synthetic list of n items
  * with k uniform removals (median of reps calls; cold = read a 1 GiB scrub
  * buffer before each call) and installs the fastest for (elem_bytes, log2 k).
- * Allocates and frees its own inputs; synchronises. */
+ * A candidate whose survivors differ from the three-kernel form's, or that
+ * sets a device error bit, is rejected: all_ms gets -2 for it (-1 for a shape
+ * the device refuses).  Allocates and frees its own inputs; synchronises. */
 rz_status rz_tune(rz_ctx* ctx, uint32_t elem_bytes, int64_t n, int64_t k, const rz_tile_config* candidates,
                   int32_t ncand, int32_t reps, int32_t cold, rz_tile_config* best, float* best_ms, float* all_ms);
 /* The same for batches: nlists lists, n log-uniform in [n_min, n_max],
- * k = floor(r n) with r uniform in [0, r_max]. */
+ * k = floor(r n) with r uniform in [0, r_max]; every list's survivors are
+ * checked against the three-kernel form's. */
 rz_status rz_tune_batch(rz_ctx* ctx, uint32_t elem_bytes, int32_t nlists, int64_t n_min, int64_t n_max,
                         double r_max, int32_t reps, const rz_tile_config* candidates, int32_t ncand,
                         rz_tile_config* best, float* best_ms, float* all_ms);

@@ -14,13 +14,15 @@ import ctypes
 import os
 
 from . import _native
-from ._native import (RZ_COOPERATIVE, RZ_METHOD_DENSE, RZ_METHOD_ORDERED, RZ_METHOD_PARTITION, RZ_METHOD_REDZONE,
-                      RZ_ANY_PAIRING, RZ_PRESERVE_REMOVALS, RZ_PROFILE, RZ_SORTED, RZ_STATS, RZ_VALIDATE, BatchInfo, ListDesc,
+from ._native import (RZ_COOPERATIVE, RZ_EDEVICE, RZ_ERR_LOOKBACK_TIMEOUT, RZ_ERR_ORPHAN_MISMATCH, RZ_ERR_WAIT_TIMEOUT,
+                      RZ_METHOD_DENSE, RZ_METHOD_ORDERED, RZ_METHOD_PARTITION, RZ_METHOD_REDZONE, RZ_ANY_PAIRING,
+                      RZ_PRESERVE_REMOVALS, RZ_PROFILE, RZ_SORTED, RZ_STATS, RZ_VALIDATE, BatchInfo, ListDesc,
                       RedzoneError, Stats, TileConfig)
 
 __all__ = ["Context", "Batch", "ListDescs", "remove", "partition", "TileConfig", "Stats", "RedzoneError", "RZ_PRESERVE_REMOVALS",
            "RZ_VALIDATE", "RZ_COOPERATIVE", "RZ_STATS", "RZ_PROFILE", "RZ_SORTED", "RZ_ANY_PAIRING", "RZ_METHOD_REDZONE",
-           "RZ_METHOD_ORDERED", "RZ_METHOD_DENSE", "RZ_METHOD_PARTITION", "elem_bytes_of", "default_context"]
+           "RZ_METHOD_ORDERED", "RZ_METHOD_DENSE", "RZ_METHOD_PARTITION", "RZ_EDEVICE", "RZ_ERR_ORPHAN_MISMATCH",
+           "RZ_ERR_WAIT_TIMEOUT", "RZ_ERR_LOOKBACK_TIMEOUT", "elem_bytes_of", "default_context"]
 
 
 def elem_bytes_of(data):
@@ -222,6 +224,22 @@ class Context:
         out = Stats()
         self._call("rz_last_stats", self._h, ctypes.byref(out))
         return out
+
+    def check_errors(self, stream=None):
+        """Waits for the stream and reads (and clears) the device error word.
+        Returns 0, or raises RedzoneError(RZ_EDEVICE) carrying the RZ_ERR_*
+        bits: the calls since the last check may have produced wrong arrays
+        (the library has reset its scratch)."""
+        flags = ctypes.c_uint32(0)
+        st = self._lib.rz_check_errors(self._h, _stream_handle(stream, self.device), ctypes.byref(flags))
+        if st != 0:
+            raise RedzoneError(st, "rz_check_errors", flags.value)
+        return 0
+
+    def debug_set_waits(self, spin_cap=0, stall=False):
+        """Test hook: in-kernel waits give up after spin_cap polls (0 = their own
+        limits); with stall the Redzone waits never see their condition met."""
+        self._call("rz_debug_set_waits", self._h, int(spin_cap), 1 if stall else 0)
 
     # ---- synthetic inputs --------------------------------------------------
     def gen_uniform_removals(self, removals, n, seed, stream=None):

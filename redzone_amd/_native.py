@@ -10,7 +10,11 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librz_amd.so")
 
 # status codes (rz_status)
-RZ_OK, RZ_EINVAL, RZ_ERANGE, RZ_EDUP, RZ_ECOOP, RZ_EHIP, RZ_ENOMEM, RZ_ENOTUNED = range(8)
+RZ_OK, RZ_EINVAL, RZ_ERANGE, RZ_EDUP, RZ_ECOOP, RZ_EHIP, RZ_ENOMEM, RZ_ENOTUNED, RZ_EDEVICE = range(9)
+# bits of the device error word (RZ_ERR_*)
+RZ_ERR_ORPHAN_MISMATCH = 0x1
+RZ_ERR_WAIT_TIMEOUT = 0x2
+RZ_ERR_LOOKBACK_TIMEOUT = 0x4
 # flags
 RZ_PRESERVE_REMOVALS = 0x1
 RZ_VALIDATE = 0x2
@@ -74,9 +78,14 @@ class Stats(ctypes.Structure):
 
 
 class RedzoneError(RuntimeError):
-    def __init__(self, status, what=""):
+    """A non-OK rz_status; error_flags holds the device error word's RZ_ERR_* bits for RZ_EDEVICE."""
+
+    def __init__(self, status, what="", error_flags=0):
         self.status = status
+        self.error_flags = error_flags
         msg = _lib.rz_status_string(status).decode() if _lib is not None else str(status)
+        if error_flags:
+            msg += f", error bits 0x{error_flags:x}"
         super().__init__(f"{what}: {msg} (status {status})" if what else f"{msg} (status {status})")
 
 
@@ -129,6 +138,8 @@ SIGNATURES = [
     ("rz_dispatch_method_config", _i32, [_i32, _u32, ctypes.POINTER(TileConfig)]),
     ("rz_dispatch_set_params", _i32, [_vp, _vp]),
     ("rz_remove_auto", _i32, [_vp, _vp, _i64, _u32, _vp, _i64, _u32, ctypes.POINTER(_i32), _vp]),
+    ("rz_check_errors", _i32, [_vp, _vp, ctypes.POINTER(_u32)]),
+    ("rz_debug_set_waits", _i32, [_vp, _u32, _u32]),
 ]
 
 TUNED_TABLE = os.path.join(_HERE, "tuned_gfx950.txt")

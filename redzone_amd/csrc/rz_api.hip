@@ -19,23 +19,70 @@ rz_status fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s) {
 }
 
 // Grow a scratch buffer.  New memory is zeroed synchronously; the kernels keep
-// it zero.  Growing frees the old buffer (hipFree waits for the device), so do
-// it outside stream capture: rz_reserve exists for callers that capture.
+// it zero.  Growing frees the old buffer, so it must not happen under stream
+// capture.  A buffer of the context may have been used on any stream (the
+// mixed path runs calls on two), so the whole device is drained before the
+// old buffer is freed, and the buffer is detached from the context before
+// hipFree is called: a failed free can never leave a pointer that a later
+// call would free again.
 rz_status ensure(rz_ctx* ctx, DeviceBuffer& b, size_t bytes, hipStream_t s) {
   (void)ctx;
+  (void)s;
   if (bytes <= b.bytes) return RZ_OK;
   size_t cap = b.bytes ? b.bytes : 4096;
   while (cap < bytes) cap *= 2;
-  RZ_HIP(hipStreamSynchronize(s));
-  if (b.ptr) RZ_HIP(hipFree(b.ptr));
+  RZ_HIP(hipDeviceSynchronize());
+  void* old = b.ptr;
   b.ptr = nullptr;
   b.bytes = 0;
+  if (old) RZ_HIP(hipFree(old));
   void* p = nullptr;
   if (hipMalloc(&p, cap) != hipSuccess) return RZ_ENOMEM;
-  RZ_HIP(hipMemset(p, 0, cap));
+  if (hipMemset(p, 0, cap) != hipSuccess) {
+    (void)hipFree(p);
+    return RZ_EHIP;
+  }
   b.ptr = p;
   b.bytes = cap;
   return RZ_OK;
+}
+
+rz_status reset_scratch(rz_ctx* ctx) {
+  RZ_HIP(hipDeviceSynchronize());
+  DeviceBuffer* bufs[] = {&ctx->tail_bits, &ctx->list_bits,  &ctx->counts,    &ctx->orphans,    &ctx->status,
+                          &ctx->queue,     &ctx->mstatus,    &ctx->slots,     &ctx->dense_cnt,  &ctx->dense_small,
+                          &ctx->dense_ent, &ctx->dense_fine, &ctx->dense_src, &ctx->gstatus,    &ctx->ploaded,
+                          &ctx->pgcnt,     &ctx->pgdone};
+  for (DeviceBuffer* b : bufs)
+    if (b->ptr) RZ_HIP(hipMemset(b->ptr, 0, b->bytes));
+  // every state word (epochs, counters, ready words, the error word); the
+  // test hooks behind them stay as the host set them
+  RZ_HIP(hipMemset(ctx->ctrl.ptr, 0, (size_t)kCtrlSpinCap * 4));
+  rz_status st;
+  if (ctx->last_batch && (st = batch_reset_scratch(ctx->last_batch)) != RZ_OK) return st;
+  for (rz_batch* b : ctx->batches)
+    if ((st = batch_reset_scratch(b)) != RZ_OK) return st;
+  RZ_HIP(hipDeviceSynchronize());
+  return RZ_OK;
+}
+
+rz_status read_errors(rz_ctx* ctx, hipStream_t s, bool all, uint32_t* error_flags) {
+  if (all)
+    RZ_HIP(hipMemcpyAsync(ctx->ctrl_host, ctx->ctrl.ptr, kCtrlWords * 4, hipMemcpyDeviceToHost, s));
+  else
+    RZ_HIP(hipMemcpyAsync(ctx->ctrl_host + kCtrlError, (uint32_t*)ctx->ctrl.ptr + kCtrlError, 4,
+                          hipMemcpyDeviceToHost, s));
+  RZ_HIP(hipStreamSynchronize(s));
+  const uint32_t e = ctx->ctrl_host[kCtrlError];
+  if (error_flags) *error_flags = e;
+  if (!e) return RZ_OK;
+  const rz_status st = reset_scratch(ctx);
+  return st != RZ_OK ? st : RZ_EDEVICE;
+}
+
+rz_status finish_call(rz_ctx* ctx, uint32_t flags, hipStream_t s) {
+  if (!(flags & (RZ_STATS | RZ_VALIDATE))) return RZ_OK;
+  return read_errors(ctx, s, (flags & RZ_STATS) != 0, nullptr);
 }
 
 rz_status prof_mark(rz_ctx* ctx, uint32_t flags, int i, hipStream_t s) {
@@ -124,6 +171,9 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
   if (!ctx) return RZ_OK;
   DeviceGuard g(ctx->device);
   (void)hipDeviceSynchronize();
+  // batches the caller has not destroyed no longer launch (RZ_EINVAL) but can still be destroyed
+  for (rz_batch* b : ctx->batches) batch_detach(b);
+  ctx->batches.clear();
   if (ctx->last_batch) rz_batch_destroy(ctx->last_batch);
   DeviceBuffer* bufs[] = {&ctx->tail_bits, &ctx->list_bits, &ctx->counts, &ctx->orphans,
                           &ctx->status,    &ctx->ctrl,      &ctx->queue,  &ctx->mstatus, &ctx->slots,
@@ -150,6 +200,7 @@ const char* rz_status_string(rz_status s) {
     case RZ_EHIP: return "HIP runtime error";
     case RZ_ENOMEM: return "out of device memory";
     case RZ_ENOTUNED: return "no model loaded";
+    case RZ_EDEVICE: return "a check on the device failed (output unspecified; scratch reset)";
   }
   return "unknown status";
 }
@@ -177,11 +228,8 @@ rz_status rz_remove_config(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_b
     if ((st = launch_validate(ctx, d_removals, n, k, flags, s)) != RZ_OK) return st;
   }
   st = launch_plain(ctx, d_data, n, elem_bytes, d_removals, k, flags, *cfg, s);
-  if (st == RZ_OK && (flags & RZ_STATS)) {
-    RZ_HIP(hipMemcpyAsync(ctx->ctrl_host, ctx->ctrl.ptr, kCtrlWords * 4, hipMemcpyDeviceToHost, s));
-    RZ_HIP(hipStreamSynchronize(s));
-  }
-  return st;
+  if (st != RZ_OK) return st;
+  return finish_call(ctx, flags, s);
 }
 
 rz_status rz_remove(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, int32_t* d_removals, int64_t k,
@@ -210,11 +258,8 @@ rz_status rz_remove_premarked(rz_ctx* ctx, void* d_data, int64_t n, uint32_t ele
   const rz_tile_config c = cfg ? *cfg : ctx->table[esz_index(elem_bytes)][log2_floor(k)];
   st = launch_plain(ctx, d_data, n, elem_bytes, d_removals, k, (flags & ~(uint32_t)RZ_SORTED) | kFlagPremarked, c, s,
                     d_tail_bits);
-  if (st == RZ_OK && (flags & RZ_STATS)) {
-    RZ_HIP(hipMemcpyAsync(ctx->ctrl_host, ctx->ctrl.ptr, kCtrlWords * 4, hipMemcpyDeviceToHost, s));
-    RZ_HIP(hipStreamSynchronize(s));
-  }
-  return st;
+  if (st != RZ_OK) return st;
+  return finish_call(ctx, flags, s);
 }
 
 rz_status rz_mark_removals(rz_ctx* ctx, const int32_t* d_removals, int64_t n, int64_t k, uint32_t* d_tail_bits,
@@ -279,7 +324,25 @@ rz_status rz_partition_on(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_by
   if (flags & RZ_VALIDATE) {
     if ((st = launch_validate(ctx, d_removals, n, k, flags, s)) != RZ_OK) return st;
   }
-  return launch_partition(ctx, d_data, n, elem_bytes, d_removals, k, flags, s);
+  if ((st = launch_partition(ctx, d_data, n, elem_bytes, d_removals, k, flags, s)) != RZ_OK) return st;
+  return finish_call(ctx, flags, s);
+}
+
+rz_status rz_check_errors(rz_ctx* ctx, void* stream, uint32_t* error_flags) {
+  if (!ctx) return RZ_EINVAL;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  DeviceGuard g(ctx->device);
+  return read_errors(ctx, (hipStream_t)stream, false, error_flags);
+}
+
+rz_status rz_debug_set_waits(rz_ctx* ctx, uint32_t spin_cap, uint32_t stall) {
+  if (!ctx) return RZ_EINVAL;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  DeviceGuard g(ctx->device);
+  const uint32_t w[2] = {spin_cap, stall};
+  RZ_HIP(hipDeviceSynchronize());
+  RZ_HIP(hipMemcpy((uint32_t*)ctx->ctrl.ptr + kCtrlSpinCap, w, sizeof(w), hipMemcpyHostToDevice));
+  return RZ_OK;
 }
 
 }  // extern "C"

@@ -78,7 +78,7 @@ __device__ __forceinline__ void run_item(const ListPlan& P, uint32_t li, uint32_
     move_slice<T, WG, IPT, true>(A, R, Rd, P.k, P.sZ, P.L, 0, lb, lc, msh);
     wait_all_memory();
     __syncthreads();
-    pair_body<T, WG, kQueueMaxSlices, true>(A, Rd, P.sZ, P.L, 1, lc, ctrl, 0, 0, 1, psh);
+    pair_body<T, WG, kQueueMaxSlices, true>(A, Rd, P.k, P.sZ, P.L, 1, lc, ctrl, 0, 0, 1, psh);
   } else if (kind == kMark) {
     const uint32_t a = j * P.L, b = min(a + P.L, P.k);
     mark_body<false>(R + a, b - a, P.k, P.sZ, lb, nullptr, tid, WG);
@@ -88,7 +88,7 @@ __device__ __forceinline__ void run_item(const ListPlan& P, uint32_t li, uint32_
     move_slice<T, WG, IPT, true>(A, R, Rd, P.k, P.sZ, P.L, j, lb, lc, msh, &done[2 * li + 1]);
   } else {
     wait_count(&done[2 * li + 1], P.m, ctrl);
-    pair_body<T, WG, kQueueMaxSlices, true>(A, Rd, P.sZ, P.L, P.m, lc, ctrl, 0, j, P.npair, psh);
+    pair_body<T, WG, kQueueMaxSlices, true>(A, Rd, P.k, P.sZ, P.L, P.m, lc, ctrl, 0, j, P.npair, psh);
   }
 }
 
@@ -426,6 +426,17 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
   return RZ_OK;
 }
 
+// After a failed device check: the batch's bitmap, slice counts and queue
+// words back to zero (the launch keeps them zero when every wait succeeds).
+rz_status rz::batch_reset_scratch(rz_batch* b) {
+  DeviceBuffer* bufs[] = {&b->qctl, &b->bits, &b->counts};
+  for (DeviceBuffer* d : bufs)
+    if (d->ptr) RZ_HIP(hipMemset(d->ptr, 0, d->bytes));
+  return RZ_OK;
+}
+
+void rz::batch_detach(rz_batch* b) { b->ctx = nullptr; }
+
 // Plans into the context's reusable batch and launches it (rz_remove_batched).
 rz_status rz::batch_plan_and_launch(rz_ctx* ctx, const rz_list_desc* lists, int32_t nlists, uint32_t flags,
                                 hipStream_t s) {
@@ -458,12 +469,16 @@ rz_status rz_batch_create(rz_ctx* ctx, const rz_list_desc* lists, int32_t nlists
     rz_batch_destroy(b);
     return st;
   }
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->batches.push_back(b);
+  }
   *out = b;
   return RZ_OK;
 }
 
 rz_status rz_batch_launch(rz_batch* b, void* stream) {
-  if (!b) return RZ_EINVAL;
+  if (!b || !b->ctx) return RZ_EINVAL;
   if (b->nitems == 0) return RZ_OK;
   hipStream_t s = (hipStream_t)stream;
   if (b->exec) {
@@ -480,7 +495,7 @@ rz_status rz_batch_launch(rz_batch* b, void* stream) {
 }
 
 rz_status rz_batch_capture(rz_batch* b, void* stream) {
-  if (!b) return RZ_EINVAL;
+  if (!b || !b->ctx) return RZ_EINVAL;
   hipStream_t s = (hipStream_t)stream;
   if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);
@@ -511,6 +526,11 @@ rz_status rz_batch_get_info(rz_batch* b, rz_batch_info* out) {
 
 rz_status rz_batch_destroy(rz_batch* b) {
   if (!b) return RZ_OK;
+  if (b->ctx && b->ctx->last_batch != b) {
+    std::lock_guard<std::mutex> lock(b->ctx->mu);
+    std::vector<rz_batch*>& v = b->ctx->batches;
+    v.erase(std::remove(v.begin(), v.end(), b), v.end());
+  }
   (void)hipDeviceSynchronize();
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include <mutex>
+#include <vector>
 
 #include "../../include/redzone.h"
 
@@ -25,14 +26,26 @@ enum CtrlWord : int {
   kCtrlMarkDone = 256,    // mark workgroups of the one-launch form that are done (reset by the last of them)
   kCtrlMarkReady = 288,   // 8 copies, one line apart (one per XCD): move-pair epoch + 1 once every mark is in
   kCtrlGatherExit = 544,  // tiles of the one-pass gather that have finished (reset by the last)
-  kCtrlWords = 576
+  kCtrlSpinCap = 576,     // test hook: polls before a bounded wait gives up (0 = the wait's own limit)
+  kCtrlStall = 577,       // test hook: nonzero makes the Redzone waits see their condition as unmet
+  kCtrlWords = 608
 };
 
 enum ErrorBits : uint32_t {
-  kErrOrphanMismatch = 1u,  // dest and source orphan totals differ (duplicate or bad input)
-  kErrBarrierTimeout = 2u,  // an in-kernel wait gave up (workgroups not co-resident)
-  kErrLookbackTimeout = 4u, // a look-back wait gave up
+  kErrOrphanMismatch = RZ_ERR_ORPHAN_MISMATCH,   // dest and source orphan totals differ (duplicate or bad input)
+  kErrBarrierTimeout = RZ_ERR_WAIT_TIMEOUT,      // an in-kernel wait gave up (workgroups not co-resident)
+  kErrLookbackTimeout = RZ_ERR_LOOKBACK_TIMEOUT, // a look-back wait gave up
 };
+
+// Polls a bounded wait makes before it gives up: the control block's cap when
+// the host set one, else the wait's own limit.
+__device__ __forceinline__ uint32_t spin_cap(const uint32_t* ctrl, uint32_t own) {
+  const uint32_t c = __hip_atomic_load(&ctrl[kCtrlSpinCap], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return c ? c : own;
+}
+__device__ __forceinline__ bool stalled(const uint32_t* ctrl) {
+  return __hip_atomic_load(&ctrl[kCtrlStall], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+}
 
 // internal flag: the tail bitmap was set by the caller (rz_remove_premarked)
 constexpr uint32_t kFlagPremarked = 0x80000000u;
@@ -77,6 +90,7 @@ struct rz_ctx {
   uint8_t* auto_host = nullptr;  // pinned staging of the descriptor and the choice
   rz_tile_config batch_cfg;    // default launch shape of batches
   rz_batch* last_batch = nullptr;  // plan of the last rz_remove_batched call
+  std::vector<rz_batch*> batches;  // batches made by rz_batch_create (their scratch is reset with the context's)
   uint32_t* ctrl_host = nullptr;  // pinned mirror for stats / validation reads
   rz_tile_config table[3][33];    // [element size 4/8/16][floor(log2 k)]
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // RZ_PROFILE marks
@@ -121,6 +135,17 @@ bool dense_bucket_marks(const rz_tile_config& cfg, int64_t k, uint32_t flags);
 bool bucket_marks_fit(int64_t m);
 rz_status bucket_marks(rz_ctx* ctx, const int32_t* R, int64_t k, uint32_t base, int64_t m, uint32_t* bits,
                        bool zero_bits, hipStream_t s);
+// After a failed device check: zero the scratch the kernels keep zero between
+// calls (context buffers, state words of the control block, batches), so that
+// the next call starts clean.  Synchronises the device.
+rz_status reset_scratch(rz_ctx* ctx);
+rz_status batch_reset_scratch(rz_batch* b);
+void batch_detach(rz_batch* b);  // the context is being destroyed
+// Reads the error word into ctrl_host (all of the control block when `all`),
+// waits for the stream, and returns RZ_EDEVICE (after reset_scratch) if a bit is set.
+rz_status read_errors(rz_ctx* ctx, hipStream_t s, bool all, uint32_t* error_flags);
+// RZ_STATS / RZ_VALIDATE: the checking read at the end of a call
+rz_status finish_call(rz_ctx* ctx, uint32_t flags, hipStream_t s);
 // RZ_PROFILE: record mark i (0..3) on the stream when profiling
 rz_status prof_mark(rz_ctx* ctx, uint32_t flags, int i, hipStream_t s);
 // fill `count` uint32 words with `v` by a kernel (graph-capture safe, unlike hipMemsetAsync)

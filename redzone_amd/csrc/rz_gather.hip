@@ -93,7 +93,7 @@ __global__ __launch_bounds__(WG) void rz_gather_inplace(T* __restrict__ A, uint3
         if (c < total) ok &= (c < g1 ? ld_relaxed(&gdone[c]) : ld_relaxed(&loaded[tb + (c - g1)])) == epoch;
       }
       if (ballot(!ok)) {
-        if (++spins > (1u << 22)) {
+        if (++spins > spin_cap(ctrl, 1u << 22)) {
           if (lane == 0) or_relaxed(&ctrl[kCtrlError], kErrLookbackTimeout);
           break;
         }
@@ -177,6 +177,7 @@ rz_status rz_gather_preserve(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem
       default: st = gather_form<uint4, 4>(ctx, d_data, n, d_keep, p, s); break;
     }
   }
+  if (st == RZ_OK) st = finish_call(ctx, flags, s);
   if (prev >= 0 && prev != ctx->device) (void)hipSetDevice(prev);
   return st;
 }

@@ -119,7 +119,7 @@ __global__ __launch_bounds__(WG) void rz_compact(T* __restrict__ A, uint32_t n, 
         // lanes up to and including the nearest inclusive prefix must all be ready
         const uint64_t upto = incl_m ? ((incl_m & (~incl_m + 1)) << 1) - 1 : ~0ull;
         if (none_m & upto) {
-          if (++spins > (1u << 24)) {  // give up (the grid is not making progress)
+          if (++spins > spin_cap(ctrl, 1u << 24)) {  // give up (the grid is not making progress)
             if (lane == 0) or_relaxed(&ctrl[kCtrlError], kErrLookbackTimeout);
             break;
           }
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(WG) void rz_compact_vec(T* __restrict__ A, uint32_t
       uint32_t pause = 1, spins = 0;
       while (true) {
         const uint64_t st = ld_relaxed(&status[t - 1]);
-        if ((((uint32_t)(st >> 32) & 0x3fffffffu) == epoch && (st >> 62) != 0u) || ++spins > (1u << 22)) break;
+        if ((((uint32_t)(st >> 32) & 0x3fffffffu) == epoch && (st >> 62) != 0u) || ++spins > spin_cap(ctrl, 1u << 22)) break;
         for (uint32_t i = 0; i < pause; ++i) __builtin_amdgcn_s_sleep(1);
         if (pause < 16u) pause <<= 1;
       }
@@ -347,7 +347,7 @@ __global__ __launch_bounds__(WG) void rz_compact_vec(T* __restrict__ A, uint32_t
         const uint64_t none_m = ballot(flag == 0u);
         const uint64_t upto = incl_m ? ((incl_m & (~incl_m + 1)) << 1) - 1 : ~0ull;
         if (none_m & upto) {
-          if (++spins > (1u << 22)) {
+          if (++spins > spin_cap(ctrl, 1u << 22)) {
             if (lane == 0) or_relaxed(&ctrl[kCtrlError], kErrLookbackTimeout);
             break;
           }
@@ -382,7 +382,7 @@ __global__ __launch_bounds__(WG) void rz_compact_vec(T* __restrict__ A, uint32_t
         else if (c < total)
           ok = ld_relaxed(&loaded[b_beg + (c - na - ng)]) == epoch;
         if (ballot(!ok)) {
-          if (++spins > (1u << 22)) {
+          if (++spins > spin_cap(ctrl, 1u << 22)) {
             if (lane == 0) or_relaxed(&ctrl[kCtrlError], kErrLookbackTimeout);
             break;
           }

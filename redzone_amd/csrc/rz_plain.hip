@@ -82,11 +82,11 @@ __global__ __launch_bounds__(WG) void rz_move(T* __restrict__ A, const int32_t* 
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void rz_pair(T* __restrict__ A, const int32_t* __restrict__ Rd, uint32_t sZ,
+__global__ __launch_bounds__(256) void rz_pair(T* __restrict__ A, const int32_t* __restrict__ Rd, uint32_t k, uint32_t sZ,
                                                 uint32_t L, uint32_t ns, const uint64_t* __restrict__ counts,
                                                 uint32_t* __restrict__ ctrl, int stats) {
   __shared__ PairShared<256, kMaxSlices> sh;
-  pair_body<T, 256, kMaxSlices, false>(A, Rd, sZ, L, ns, counts, ctrl, stats, blockIdx.x, gridDim.x, sh);
+  pair_body<T, 256, kMaxSlices, false>(A, Rd, k, sZ, L, ns, counts, ctrl, stats, blockIdx.x, gridDim.x, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -249,6 +249,8 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
     if (slice > 0) {
       int64_t j = (int64_t)slice - 1;
       uint32_t spins = 0;
+      const uint32_t cap = spin_cap(ctrl, 1u << 24);
+      const bool stall = stalled(ctrl);
       while (true) {
         const int64_t q = j - lane;
         const uint64_t w = q >= 0 ? ld_relaxed(&st[q]) : kStIncl;
@@ -257,8 +259,10 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
         const uint64_t none_m = ballot(flag == 0u);
         // lanes up to and including the nearest inclusive prefix must all be ready
         const uint64_t upto = incl_m ? ((incl_m & (~incl_m + 1)) << 1) - 1 : ~0ull;
-        if (none_m & upto) {
-          if (++spins > (1u << 24)) {
+        if (stall || (none_m & upto)) {
+          // a give-up leaves excl short: the ranks below stay inside the
+          // exchange words and every move below checks its indices
+          if (++spins > cap) {
             if (lane == 0) or_relaxed(&ctrl[kCtrlError], kErrLookbackTimeout);
             break;
           }
@@ -375,9 +379,10 @@ __device__ __forceinline__ void grid_barrier(uint32_t* __restrict__ ctrl, uint64
     uint64_t* cnt = (uint64_t*)&ctrl[kCtrlBarCount];
     add_relaxed(cnt, 1ull);
     uint32_t spins = 0;
+    const uint32_t cap = spin_cap(ctrl, 1u << 22);
     while (ld_relaxed(cnt) < target) {
       __builtin_amdgcn_s_sleep(2);
-      if (++spins > (1u << 22)) {
+      if (++spins > cap) {
         or_relaxed(&ctrl[kCtrlError], kErrBarrierTimeout);
         break;
       }
@@ -405,7 +410,7 @@ __global__ __launch_bounds__(WG) void rz_coop(T* __restrict__ A, const int32_t* 
   if (blockIdx.x == 0 && threadIdx.x == 0) st_relaxed((uint64_t*)&ctrl[kCtrlBarEpoch], base + 2ull * G);
   for (uint32_t s = blockIdx.x; s < ns; s += G) move_slice<T, WG, IPT, true>(A, R, Rd, k, sZ, L, s, bits, counts, msh);
   grid_barrier(ctrl, base + 2ull * G);
-  pair_body<T, WG, kMaxSlices, true>(A, Rd, sZ, L, ns, counts, ctrl, stats, blockIdx.x, G, psh);
+  pair_body<T, WG, kMaxSlices, true>(A, Rd, k, sZ, L, ns, counts, ctrl, stats, blockIdx.x, G, psh);
 }
 
 // ---------------------------------------------------------------------------
@@ -443,7 +448,7 @@ __global__ __launch_bounds__(WG) void rz_ordered(T* __restrict__ A, const int32_
     move_slice<T, WG, IPT, true>(A, R, Rd, k, sZ, L, b - nm, bits, counts, msh, &q[kOrdMove]);
   } else {
     wait_count(&q[kOrdMove], ns, ctrl);
-    pair_body_lds<T, WG, true>(A, Rd, sZ, L, ns, counts, ctrl, stats, b - nm - ns, gridDim.x - nm - ns, dyn,
+    pair_body_lds<T, WG, true>(A, Rd, k, sZ, L, ns, counts, ctrl, stats, b - nm - ns, gridDim.x - nm - ns, dyn,
                                dyn + ns + 1);
   }
   if (tid == 0) {
@@ -609,9 +614,9 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
   uint64_t* counts = (uint64_t*)ctx->counts.ptr;
   uint32_t* ctrl = (uint32_t*)ctx->ctrl.ptr;
   const int stats = (flags & RZ_STATS) ? 1 : 0;
-  if (stats) {  // RZ_STATS reports the counts and the error bits of this call only
+  if (stats) {  // RZ_STATS reports the counts of this call only (the error word is cleared when it is read)
     if ((st = fill_u32(ctrl + kCtrlStatTail, 0u, 2, s)) != RZ_OK) return st;
-    if ((st = fill_u32(ctrl + kCtrlError, 0u, 1, s)) != RZ_OK) return st;
+    if ((st = fill_u32(ctrl + kCtrlStatOrphans, 0u, 2, s)) != RZ_OK) return st;
   }
   if ((st = prof_mark(ctx, flags, 0, s)) != RZ_OK) return st;
   if (coop || cfg.variant == 3) {
@@ -677,7 +682,7 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
   RZ_HIP(hipGetLastError());
   if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;
   const dim3 g3((uint32_t)ctx->cu_count);
-  hipLaunchKernelGGL(rz_pair<T>, g3, dim3(256), 0, s, A, Rd, sZ, L, ns, counts, ctrl, stats);
+  hipLaunchKernelGGL(rz_pair<T>, g3, dim3(256), 0, s, A, Rd, (uint32_t)k, sZ, L, ns, counts, ctrl, stats);
   RZ_HIP(hipGetLastError());
   return prof_mark(ctx, flags, 3, s);
 }

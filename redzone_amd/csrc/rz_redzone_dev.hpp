@@ -218,8 +218,8 @@ struct PairShared {
 // s_tmp: WG/64+1 entries), then pairs t = block*WG + tid, stepping by
 // nblocks*WG.  Dest orphans are the low 32 bits of a count word.
 template <typename T, int WG, bool COH>
-__device__ __forceinline__ void pair_body_lds(T* __restrict__ A, const int32_t* __restrict__ Rd, uint32_t sZ,
-                                              uint32_t L, uint32_t ns, const uint64_t* __restrict__ counts,
+__device__ __forceinline__ void pair_body_lds(T* __restrict__ A, const int32_t* __restrict__ Rd, uint32_t k,
+                                              uint32_t sZ, uint32_t L, uint32_t ns, const uint64_t* __restrict__ counts,
                                               uint32_t* __restrict__ ctrl, int stats, uint32_t block,
                                               uint32_t nblocks, uint64_t* s_pre, uint64_t* s_tmp) {
   const int tid = threadIdx.x;
@@ -257,18 +257,24 @@ __device__ __forceinline__ void pair_body_lds(T* __restrict__ A, const int32_t* 
   for (uint32_t t = block * WG + tid; t < pairs; t += nblocks * WG) {
     const int sd = upper_row((int)ns, t, [&](int s) { return (uint32_t)s_pre[s]; });
     const int ss = upper_row((int)ns, t, [&](int s) { return (uint32_t)(s_pre[s] >> 32); });
-    const uint32_t dst = ld_x<COH>((const uint32_t*)&Rd[sd * L + (t - (uint32_t)s_pre[sd])]);
-    A[dst] = ld_x<COH>(&Z[ss * L + (t - (uint32_t)(s_pre[ss] >> 32))]);
+    // With valid counts both offsets lie inside their slice; the guards keep a
+    // wait that gave up (stale counts) from indexing outside R, Z or the head.
+    const uint32_t od = t - (uint32_t)s_pre[sd], os = t - (uint32_t)(s_pre[ss] >> 32);
+    if (od >= L || os >= L) continue;
+    const uint32_t id = (uint32_t)sd * L + od, is = (uint32_t)ss * L + os;
+    if (id >= k || is >= k) continue;
+    const uint32_t dst = ld_x<COH>((const uint32_t*)&Rd[id]);
+    if (dst < sZ) A[dst] = ld_x<COH>(&Z[is]);
   }
   __syncthreads();  // s_pre may be reused by the caller
 }
 
 template <typename T, int WG, int MAXS, bool COH>
-__device__ __forceinline__ void pair_body(T* __restrict__ A, const int32_t* __restrict__ Rd, uint32_t sZ,
+__device__ __forceinline__ void pair_body(T* __restrict__ A, const int32_t* __restrict__ Rd, uint32_t k, uint32_t sZ,
                                           uint32_t L, uint32_t ns, const uint64_t* __restrict__ counts,
                                           uint32_t* __restrict__ ctrl, int stats, uint32_t block,
                                           uint32_t nblocks, PairShared<WG, MAXS>& sh) {
-  pair_body_lds<T, WG, COH>(A, Rd, sZ, L, ns, counts, ctrl, stats, block, nblocks, sh.pre, sh.tmp);
+  pair_body_lds<T, WG, COH>(A, Rd, k, sZ, L, ns, counts, ctrl, stats, block, nblocks, sh.pre, sh.tmp);
 }
 
 // ---------------------------------------------------------------------------
@@ -282,13 +288,16 @@ __device__ __forceinline__ void signal_done(uint32_t* counter) {
 }
 
 // Wait (thread 0 polls, the workgroup waits at a barrier) until the counter
-// reaches target; a bounded wait that flags kErrBarrierTimeout if it gives up.
+// reaches target; a bounded wait that flags kErrBarrierTimeout if it gives up
+// (after spin_cap polls; the stall hook makes every poll fail).
 __device__ __forceinline__ void wait_count(const uint32_t* counter, uint32_t target, uint32_t* ctrl) {
   if (threadIdx.x == 0) {
     uint32_t spins = 0;
-    while (ld_relaxed(counter) < target) {
+    const uint32_t cap = spin_cap(ctrl, 1u << 22);
+    const bool stall = stalled(ctrl);
+    while (stall || ld_relaxed(counter) < target) {
       __builtin_amdgcn_s_sleep(1);
-      if (++spins > (1u << 22)) {
+      if (++spins > cap) {
         or_relaxed(&ctrl[kCtrlError], kErrBarrierTimeout);
         break;
       }
@@ -304,10 +313,12 @@ __device__ __forceinline__ void wait_value_backoff(const uint32_t* word, uint32_
                                                    uint32_t cap) {
   if (threadIdx.x == 0) {
     uint32_t spins = 0, pause = 1;
-    while (ld_relaxed(word) != value) {
+    const uint32_t limit = spin_cap(ctrl, 1u << 22);
+    const bool stall = stalled(ctrl);
+    while (stall || ld_relaxed(word) != value) {
       for (uint32_t i = 0; i < pause; ++i) __builtin_amdgcn_s_sleep(1);
       if (pause < cap) pause <<= 1;
-      if (++spins > (1u << 22)) {
+      if (++spins > limit) {
         or_relaxed(&ctrl[kCtrlError], kErrBarrierTimeout);
         break;
       }

@@ -31,6 +31,21 @@ __global__ __launch_bounds__(256) void rz_scrub_kernel(const uint4* __restrict__
   if (acc == 0x9e3779b9u && lane_id() == 0) sink[0] = acc;  // keeps the loads alive; practically never true
 }
 
+// Counts the 4-byte words that differ between a and b inside the ranges
+// (word offset, word count) into *mismatches.
+__global__ __launch_bounds__(256) void rz_compare_ranges(const uint32_t* __restrict__ a,
+                                                          const uint32_t* __restrict__ b,
+                                                          const uint64_t* __restrict__ ranges, uint32_t nranges,
+                                                          uint32_t* __restrict__ mismatches) {
+  uint32_t bad = 0;
+  for (uint32_t r = blockIdx.x; r < nranges; r += gridDim.x) {
+    const uint64_t off = ranges[2 * r], cnt = ranges[2 * r + 1];
+    for (uint64_t i = threadIdx.x; i < cnt; i += 256) bad += a[off + i] != b[off + i] ? 1u : 0u;
+  }
+  bad = wave_sum(bad);
+  if (lane_id() == 0 && bad) atomicAdd(mismatches, bad);
+}
+
 struct Scratch {
   std::vector<void*> ptrs;
   ~Scratch() {
@@ -43,6 +58,26 @@ struct Scratch {
     return p;
   }
 };
+
+// Whether a candidate's last call left the same survivors as the reference
+// result and no device error: the word ranges are compared on the device, and
+// the context's error word is read (and, if set, cleared with the scratch).
+static rz_status candidate_ok(rz_ctx* ctx, const void* got, const void* ref, const uint64_t* d_ranges,
+                              uint32_t nranges, uint32_t* d_count, hipStream_t s, bool* ok) {
+  *ok = false;
+  RZ_HIP(hipMemsetAsync(d_count, 0, 4, s));
+  hipLaunchKernelGGL(rz_compare_ranges, dim3(std::min<uint32_t>(nranges, 4096u)), dim3(256), 0, s,
+                     (const uint32_t*)got, (const uint32_t*)ref, d_ranges, nranges, d_count);
+  RZ_HIP(hipGetLastError());
+  uint32_t bad = 0;
+  RZ_HIP(hipMemcpyAsync(&bad, d_count, 4, hipMemcpyDeviceToHost, s));
+  RZ_HIP(hipStreamSynchronize(s));
+  uint32_t err = 0;
+  const rz_status st = rz_check_errors(ctx, s, &err);
+  if (st != RZ_OK && st != RZ_EDEVICE) return st;
+  *ok = bad == 0 && err == 0;
+  return RZ_OK;
+}
 
 static float median(std::vector<float> v) {
   std::sort(v.begin(), v.end());
@@ -154,7 +189,9 @@ rz_status rz_table_load(rz_ctx* ctx, const char* path) {
 
 // Times each candidate (median of `reps` calls after one warm-up call) on a
 // synthetic list; all_ms (ncand entries, may be null) receives the medians.
-// The fastest is installed in the table.  Allocates its own inputs.
+// A candidate whose result differs from the three-kernel form's, or that sets
+// a device error bit, is rejected (-2).  The fastest of the rest is installed
+// in the table.  Allocates its own inputs.
 rz_status rz_tune(rz_ctx* ctx, uint32_t elem_bytes, int64_t n, int64_t k, const rz_tile_config* cands,
                   int32_t ncand, int32_t reps, int32_t cold, rz_tile_config* best, float* best_ms, float* all_ms) {
   if (!ctx || esz_index(elem_bytes) < 0 || n < 1 || k < 1 || k >= n || n >= (1ll << 31) || reps < 1) return RZ_EINVAL;
@@ -175,7 +212,10 @@ rz_status rz_tune(rz_ctx* ctx, uint32_t elem_bytes, int64_t n, int64_t k, const 
   const uint64_t scrub_bytes = cold ? (1ull << 30) : 0;
   uint4* scrub = cold ? (uint4*)sc.alloc(scrub_bytes) : nullptr;
   uint32_t* sink = (uint32_t*)sc.alloc(256);
-  if (!data || !pristine || !R || !R0 || (cold && !scrub) || !sink) return RZ_ENOMEM;
+  void* ref = sc.alloc(bytes);
+  uint64_t* d_range = (uint64_t*)sc.alloc(16);
+  uint32_t* d_count = (uint32_t*)sc.alloc(256);
+  if (!data || !pristine || !R || !R0 || (cold && !scrub) || !sink || !ref || !d_range || !d_count) return RZ_ENOMEM;
   hipStream_t s;
   RZ_HIP(hipStreamCreate(&s));
   hipEvent_t e0, e1;
@@ -184,6 +224,20 @@ rz_status rz_tune(rz_ctx* ctx, uint32_t elem_bytes, int64_t n, int64_t k, const 
   rz_status st = rz_gen_payload(ctx, pristine, n, elem_bytes, 0x5eed0001ull, s);
   if (st == RZ_OK) st = rz_gen_uniform_removals(ctx, R0, n, k, 0x5eed0002ull, s);
   if (cold && st == RZ_OK) (void)hipMemsetAsync(scrub, 0x5a, scrub_bytes, s);
+  // the reference result every candidate must reproduce: the three-kernel form
+  // with the default shape
+  const uint64_t range[2] = {0, (uint64_t)(n - k) * elem_bytes / 4};
+  const rz_tile_config ref_cfg = default_config(elem_bytes, n, k);
+  if (st == RZ_OK &&
+      (hipMemcpyAsync(ref, pristine, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+       hipMemcpyAsync(R, R0, (size_t)k * 4, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+       hipMemcpyAsync(d_range, range, 16, hipMemcpyHostToDevice, s) != hipSuccess))
+    st = RZ_EHIP;
+  if (st == RZ_OK) st = rz_remove_config(ctx, ref, n, elem_bytes, R, k, 0, &ref_cfg, s);
+  if (st == RZ_OK) {
+    uint32_t err = 0;
+    st = rz_check_errors(ctx, s, &err);
+  }
   int best_i = -1;
   float best_t = 0.f;
   for (int32_t c = 0; c < ncand && st == RZ_OK; ++c) {
@@ -213,6 +267,13 @@ rz_status rz_tune(rz_ctx* ctx, uint32_t elem_bytes, int64_t n, int64_t k, const 
         continue;
       }
       break;
+    }
+    // the last call's survivors must equal the reference's, with no device error
+    bool ok = false;
+    if ((st = candidate_ok(ctx, data, ref, d_range, 1, d_count, s, &ok)) != RZ_OK) break;
+    if (!ok) {
+      if (all_ms) all_ms[c] = -2.f;
+      continue;
     }
     const float m = median(ts);
     if (all_ms) all_ms[c] = m;
@@ -267,11 +328,15 @@ rz_status rz_tune_batch(rz_ctx* ctx, uint32_t elem_bytes, int32_t nlists, int64_
   char* pristine = (char*)sc.alloc(bytes);
   int32_t* R = (int32_t*)sc.alloc((size_t)roff[nlists] * 4 + 4);
   int32_t* R0 = (int32_t*)sc.alloc((size_t)roff[nlists] * 4 + 4);
-  if (!data || !pristine || !R || !R0) return RZ_ENOMEM;
+  char* ref = (char*)sc.alloc(bytes);
+  uint64_t* d_ranges = (uint64_t*)sc.alloc((size_t)nlists * 16);
+  uint32_t* d_count = (uint32_t*)sc.alloc(256);
+  if (!data || !pristine || !R || !R0 || !ref || !d_ranges || !d_count) return RZ_ENOMEM;
   hipStream_t s;
   RZ_HIP(hipStreamCreate(&s));
   rz_status st = RZ_OK;
   std::vector<rz_list_desc> descs(nlists);
+  std::vector<uint64_t> ranges(2 * (size_t)nlists);
   for (int32_t i = 0; i < nlists && st == RZ_OK; ++i) {
     st = rz_gen_payload(ctx, pristine + (size_t)off[i] * elem_bytes, ns[i], elem_bytes, 100 + i, s);
     if (st == RZ_OK) st = rz_gen_uniform_removals(ctx, R0 + roff[i], ns[i], ks[i], 200 + i, s);
@@ -281,6 +346,22 @@ rz_status rz_tune_batch(rz_ctx* ctx, uint32_t elem_bytes, int32_t nlists, int64_
     descs[i].k = ks[i];
     descs[i].elem_bytes = elem_bytes;
     descs[i].method = 0;
+    ranges[2 * i] = (uint64_t)off[i] * elem_bytes / 4;  // the list's survivors, in words
+    ranges[2 * i + 1] = (uint64_t)(ns[i] - ks[i]) * elem_bytes / 4;
+  }
+  // the reference result: every list by the three-kernel form with the default shape
+  if (st == RZ_OK &&
+      (hipMemcpyAsync(ref, pristine, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+       hipMemcpyAsync(R, R0, (size_t)roff[nlists] * 4, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+       hipMemcpyAsync(d_ranges, ranges.data(), (size_t)nlists * 16, hipMemcpyHostToDevice, s) != hipSuccess))
+    st = RZ_EHIP;
+  for (int32_t i = 0; i < nlists && st == RZ_OK; ++i) {
+    const rz_tile_config rc = default_config(elem_bytes, ns[i], ks[i]);
+    st = rz_remove_config(ctx, ref + (size_t)off[i] * elem_bytes, ns[i], elem_bytes, R + roff[i], ks[i], 0, &rc, s);
+  }
+  if (st == RZ_OK) {
+    uint32_t err = 0;
+    st = rz_check_errors(ctx, s, &err);
   }
   hipEvent_t e0, e1;
   RZ_HIP(hipEventCreate(&e0));
@@ -308,6 +389,12 @@ rz_status rz_tune_batch(rz_ctx* ctx, uint32_t elem_bytes, int32_t nlists, int64_
     }
     rz_batch_destroy(b);
     if (st != RZ_OK) break;
+    bool ok = false;
+    if ((st = candidate_ok(ctx, data, ref, d_ranges, (uint32_t)nlists, d_count, s, &ok)) != RZ_OK) break;
+    if (!ok) {
+      if (all_ms) all_ms[c] = -2.f;
+      continue;
+    }
     const float m = median(ts);
     if (all_ms) all_ms[c] = m;
     if (best_i < 0 || m < best_t) {

@@ -19,6 +19,7 @@ Removal patterns (k distinct indices in [0, n), returned as int32):
               replacement (draw-and-discard-repeats), produced exactly by
               weighted sampling without replacement with the discrete
               Beta(5,1) masses w_p ~ (p+1)^5 - p^5 (exponential race).
+  redzone_pct the reference's red-zone-share lists (redzone_pct_removals).
 """
 import math
 
@@ -127,6 +128,33 @@ def tail_heavy_removals(n, k, seed, device="cpu"):
     return R.to(torch.int32)
 
 
+def redzone_pct_removals(n, k, red_pct, seed, device="cpu"):
+    """The reference's test lists with a red-zone share (TestLists_t::Update(n,
+    k, r), DataStruct.hxx:97-122): red_k = floor(k * r / 100) removals drawn
+    uniformly without replacement from the red zone [n-k, n) and the other
+    k - red_k from the green zone [0, n-k), listed green first, then red.
+    r = -1 is the plain uniform list over [0, n) (DataStruct.hxx:85-95).  The
+    reference sweeps r over {100, -1, 0, 25, 50, 75} (redzone.hxx:1094) and
+    skips r >= 0 when k > n/2 (redzone.hxx:1121), where the green zone is
+    smaller than k - red_k."""
+    if red_pct == -1:
+        return uniform_removals(n, k, seed, device)
+    if not 0 <= red_pct <= 100:
+        raise ValueError("red_pct is -1 or a percentage")
+    red = (k * red_pct) // 100
+    green = k - red
+    if green > n - k:
+        raise ValueError("the green zone [0, n-k) holds fewer than k - red_k positions")
+    parts = []
+    if green:
+        parts.append(permute_range(torch.arange(green, dtype=torch.int64, device=device), n - k, seed, salt=7))
+    if red:
+        parts.append((n - k) + permute_range(torch.arange(red, dtype=torch.int64, device=device), k, seed, salt=8))
+    if not parts:
+        return torch.empty(0, dtype=torch.int32, device=device)
+    return torch.cat(parts).to(torch.int32)
+
+
 PATTERNS = {"uniform": uniform_removals, "clustered": clustered_removals, "tail_heavy": tail_heavy_removals}
 
 
@@ -167,6 +195,33 @@ def rec16_payload(n, seed, device="cpu"):
     u2 = (hash32(i, seed * 29 + 6).to(torch.float64) + 0.5) / 4294967296.0
     val = (torch.sqrt(-2.0 * torch.log(u1)) * torch.cos(2.0 * math.pi * u2)).to(torch.float32)
     return torch.stack([key_lo, key_hi, tag, val.view(torch.int32)], dim=1).contiguous()
+
+
+# 32-bit words that a marking scheme could mistake for a mark: the reference's
+# GPU sentinel 0x80000000 (redzone.hxx:704, also the bits of float -0.0), its
+# O(n) marker -1 (redzone.hxx:594), zero, and float NaNs with payloads, the
+# infinities and a denormal
+EDGE_WORDS = (0x80000000, 0xFFFFFFFF, 0x00000000, 0x7FFFFFFF, 0x7FC00001, 0x7F800001, 0xFFC12345, 0x7F800000,
+              0xFF800000, 0x00000001)
+
+
+def edge_payload(elem_bytes, n, seed, device="cpu"):
+    """n items of elem_bytes (an (n, w) int32 tensor, or (n,) for 4 bytes) in
+    which every word of about half of the items is one of EDGE_WORDS (so the
+    sentinels sit in the head and in the red zone alike) and the rest are
+    unique hashes; the first word of an item is unique whenever it is not an
+    edge word.  Moved items must keep these bits exactly."""
+    w = elem_bytes // 4
+    i = torch.arange(n, dtype=torch.int64, device=device)
+    edge = torch.tensor(EDGE_WORDS, dtype=torch.int64, device=device)
+    cols = []
+    for c in range(w):
+        h = hash32(i, seed * 31 + 7 * c + 1)
+        pick = edge[h % len(EDGE_WORDS)]
+        plain = permute_bits(i, 32, seed, salt=9 + c) if c == 0 else hash32(i, seed * 37 + c)
+        cols.append(_to_int32(torch.where((h >> 8) % 2 == 0, pick, plain)))
+    out = torch.stack(cols, dim=1).contiguous()
+    return out.reshape(n) if w == 1 else out
 
 
 def payload(elem_bytes, n, seed, device="cpu"):

@@ -39,6 +39,7 @@ def gpu_remove_parity(ctx, oracle, A, R, config=None, flags=0, want=None):
     dR = R.cuda()
     new_len = ctx.remove(dA, dR, config=config, flags=flags)
     torch.cuda.synchronize()
+    assert ctx.check_errors() == 0  # raises RedzoneError(RZ_EDEVICE) if a device-side check failed
     got = dA.cpu().numpy()
     assert new_len == n - k
     m = first_mismatch(got[: n - k], want[: n - k])

@@ -44,8 +44,8 @@ def test_status_strings():
     from redzone_amd import _native
 
     lib = _native.load()
-    for s in range(8):
-        assert lib.rz_status_string(s)
+    for s in range(9):
+        assert lib.rz_status_string(s) and lib.rz_status_string(s) != b"unknown status"
     assert lib.rz_status_string(0) == b"ok"
 
 

@@ -63,3 +63,60 @@ def test_payloads_unique_and_sized():
     assert c.shape == (n, 4) and len(np.unique(c[:, 0])) == n
     vals = c[:, 3].view(np.float32)
     assert abs(vals.mean()) < 0.05 and abs(vals.std() - 1) < 0.05
+
+
+@pytest.mark.parametrize("n,frac", [(1000, 0.02), (50000, 0.1), (50001, 0.5), (4096, 0.25)])
+@pytest.mark.parametrize("r", [100, -1, 0, 25, 50, 75])
+def test_redzone_pct_lists(n, frac, r):
+    """DataStruct.hxx:97-122: floor(k r / 100) removals in the red zone
+    [n-k, n), listed after the green ones, all distinct and in range."""
+    k = int(n * frac)
+    R = W.redzone_pct_removals(n, k, r, seed=5).numpy()
+    assert R.shape == (k,) and len(np.unique(R)) == k
+    assert R.min() >= 0 and R.max() < n
+    if r >= 0:
+        red = (k * r) // 100
+        tail = R >= n - k
+        assert int(tail.sum()) == red
+        assert not tail[: k - red].any() and tail[k - red:].all()
+    assert np.array_equal(R, W.redzone_pct_removals(n, k, r, seed=5).numpy())
+
+
+def test_redzone_pct_rejects_small_green_zone():
+    with pytest.raises(ValueError):
+        W.redzone_pct_removals(1000, 900, 0, seed=1)
+
+
+@pytest.mark.parametrize("r", [0, 25, 50, 75, 100])
+def test_redzone_pct_orphans_by_construction(oracle, r):
+    """The orphan structure the red-zone share fixes: k_z = red_k; r = 0 has no
+    orphans (every hole takes its own tail item), r = 100 has no holes below
+    the red zone (the survivors stay where they are); in between the dest and
+    source orphan counts are equal (paper Lemma 4.2)."""
+    n, k = 1 << 16, 1 << 12
+    A = np.arange(n, dtype=np.int32)
+    R = W.redzone_pct_removals(n, k, r, seed=11).numpy()
+    sz = n - k
+    dead = np.zeros(k, bool)
+    dead[R[R >= sz] - sz] = True
+    dst = int(((R < sz) & dead).sum())
+    src = int(((R >= sz) & ~dead).sum())
+    assert dst == src
+    want = A.copy()
+    assert oracle.canonical(want, R.copy()) == 0
+    if r == 0:
+        assert dst == 0
+        exp = A.copy()
+        exp[R] = A[sz:]
+        assert np.array_equal(want[:sz], exp[:sz])
+    if r == 100:
+        assert np.array_equal(want[:sz], A[:sz])
+
+
+@pytest.mark.parametrize("esz", [4, 8, 16])
+def test_edge_payload_has_every_edge_word(esz):
+    n = 20000
+    P = W.edge_payload(esz, n, seed=3).numpy().reshape(n, -1).view(np.uint32)
+    for word in W.EDGE_WORDS:
+        assert (P[: n // 2] == word).any() and (P[n // 2:] == word).any()
+    assert np.array_equal(P, W.edge_payload(esz, n, seed=3).numpy().reshape(n, -1).view(np.uint32))
