@@ -43,8 +43,9 @@ typedef enum rz_status {
   RZ_EHIP = 5,     /* a HIP runtime call failed */
   RZ_ENOMEM = 6,   /* scratch allocation failed */
   RZ_ENOTUNED = 7, /* no learned model loaded where one is required */
-  RZ_EDEVICE = 8   /* a check made on the device failed (see rz_check_errors): the output of the calls since the
+  RZ_EDEVICE = 8,  /* a check made on the device failed (see rz_check_errors): the output of the calls since the
                       last check is unspecified; the context's scratch has been reset */
+  RZ_ETOOBIG = 9   /* k above the fixed scratch of rz_remove_noalloc (RZ_NOALLOC_MAX_K); nothing was run */
 } rz_status;
 
 /* flags */
@@ -339,6 +340,27 @@ rz_status rz_dispatch_set_params(rz_ctx* ctx, const float* h_params);
  * (README.md:6; paper p.14: Redzone below k = n/2, O(n) above). */
 rz_status rz_remove_auto(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, int32_t* d_removals, int64_t k,
                          uint32_t flags, int32_t* method, void* stream);
+
+/* ---- allocation-free removal -------------------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): rz_remove's contract on the current device with no
+ * context and no allocation, freeing or host synchronisation.  All scratch (a
+ * control block, look-back words, the k-bit tail bitmap and k/2 + 1 exchange
+ * words) is one fixed device array of about 16.6 MiB that every GPU the
+ * library runs on holds until the process ends, so k is limited to
+ * RZ_NOALLOC_MAX_K (RZ_ETOOBIG above it, before anything runs).  The launch is
+ * the fused form with a fixed shape by k (a prologue kernel, then the
+ * one-launch form below 2^19 removals, else the mark kernel and the fused
+ * move-and-pair kernel).  flags: RZ_PRESERVE_REMOVALS (always true here: the
+ * fused form never writes d_removals), RZ_SORTED, RZ_ANY_PAIRING; any other
+ * flag is RZ_EINVAL.  d_error_flags (may be null) is a caller-owned device
+ * word into which the call ORs its RZ_ERR_* bits, in stream order.  Calls
+ * that overlap on one GPU (other streams or threads) race on the shared
+ * array: order them. */
+#define RZ_NOALLOC_MAX_K (1 << 22)
+rz_status rz_remove_noalloc(void* d_data, int64_t n, uint32_t elem_bytes, int32_t* d_removals, int64_t k,
+                            uint32_t flags, uint32_t* d_error_flags, void* stream);
 
 #ifdef __cplusplus
 }

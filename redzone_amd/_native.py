@@ -10,7 +10,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librz_amd.so")
 
 # status codes (rz_status)
-RZ_OK, RZ_EINVAL, RZ_ERANGE, RZ_EDUP, RZ_ECOOP, RZ_EHIP, RZ_ENOMEM, RZ_ENOTUNED, RZ_EDEVICE = range(9)
+RZ_OK, RZ_EINVAL, RZ_ERANGE, RZ_EDUP, RZ_ECOOP, RZ_EHIP, RZ_ENOMEM, RZ_ENOTUNED, RZ_EDEVICE, RZ_ETOOBIG = range(10)
+RZ_NOALLOC_MAX_K = 1 << 22
 # bits of the device error word (RZ_ERR_*)
 RZ_ERR_ORPHAN_MISMATCH = 0x1
 RZ_ERR_WAIT_TIMEOUT = 0x2
@@ -140,6 +141,7 @@ SIGNATURES = [
     ("rz_remove_auto", _i32, [_vp, _vp, _i64, _u32, _vp, _i64, _u32, ctypes.POINTER(_i32), _vp]),
     ("rz_check_errors", _i32, [_vp, _vp, ctypes.POINTER(_u32)]),
     ("rz_debug_set_waits", _i32, [_vp, _u32, _u32]),
+    ("rz_remove_noalloc", _i32, [_vp, _i64, _u32, _vp, _i64, _u32, _vp, _vp]),
 ]
 
 TUNED_TABLE = os.path.join(_HERE, "tuned_gfx950.txt")

@@ -201,6 +201,7 @@ const char* rz_status_string(rz_status s) {
     case RZ_ENOMEM: return "out of device memory";
     case RZ_ENOTUNED: return "no model loaded";
     case RZ_EDEVICE: return "a check on the device failed (output unspecified; scratch reset)";
+    case RZ_ETOOBIG: return "k above the fixed scratch of the allocation-free entry point";
   }
   return "unknown status";
 }

@@ -144,6 +144,22 @@ void batch_detach(rz_batch* b);  // the context is being destroyed
 // Reads the error word into ctrl_host (all of the control block when `all`),
 // waits for the stream, and returns RZ_EDEVICE (after reset_scratch) if a bit is set.
 rz_status read_errors(rz_ctx* ctx, hipStream_t s, bool all, uint32_t* error_flags);
+// Scratch of one fused call that the caller provides (rz_remove_noalloc): the
+// control block (kCtrlWords words), two halves of kMaxSlices look-back words,
+// ceil(k/32) tail bitmap words and k/2 + 1 exchange words; err_out (may be
+// null) receives the call's error bits.
+struct FusedScratch {
+  uint32_t* ctrl;
+  uint64_t* mstatus;
+  uint32_t* bits;
+  uint64_t* slots;
+  uint32_t* err_out;
+};
+// The fused form (variant 4: mark kernel, then rz_move_pair; variant 6: one
+// launch) on that scratch, after a prologue kernel that clears the bitmap words
+// of this k, the error word and the exit counters.  Allocates nothing.
+rz_status launch_fused_static(void* data, int64_t n, uint32_t esz, const int32_t* R, int64_t k, uint32_t flags,
+                              const rz_tile_config& cfg, const FusedScratch& fs, hipStream_t s);
 // RZ_STATS / RZ_VALIDATE: the checking read at the end of a call
 rz_status finish_call(rz_ctx* ctx, uint32_t flags, hipStream_t s);
 // RZ_PROFILE: record mark i (0..3) on the stream when profiling

@@ -125,7 +125,8 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
                                                uint32_t sZ, uint32_t L, uint32_t ns, uint32_t* __restrict__ bits,
                                                uint64_t* __restrict__ mstatus, uint64_t* __restrict__ slots,
                                                uint32_t slot_cap, uint32_t* __restrict__ ctrl, int stats,
-                                               int early, uint32_t slice, uint32_t backoff) {
+                                               int early, uint32_t slice, uint32_t backoff,
+                                               uint32_t* __restrict__ err_out) {
   constexpr int NW = WG / kWave;
   constexpr int E = MoveShared<WG, IPT>::E;
   constexpr int PER = (E + kWave - 1) / kWave;
@@ -310,10 +311,19 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
   // leave the bitmap zero for the next call (slices are word aligned)
   for (uint32_t w = (a >> 5) + tid; w < ((b + 31u) >> 5); w += WG) bits[w] = 0u;
   // the last workgroup to leave (all of them have read the epoch by then)
-  // starts the next call's epoch
-  if (tid == 0 && add_relaxed(&ctrl[kCtrlMoveExit], 1u) == ns - 1u) {
-    st_relaxed(&ctrl[kCtrlMoveExit], 0u);
-    st_relaxed(&ctrl[kCtrlMoveEpoch], epoch + 1u);
+  // starts the next call's epoch.  With err_out, it also hands the call's
+  // error bits to the caller's word: every bit a workgroup sets is set by its
+  // thread 0, whose returning atomics have completed before its exit count.
+  if (tid == 0) {
+    if (err_out) wait_all_memory();
+    if (add_relaxed(&ctrl[kCtrlMoveExit], 1u) == ns - 1u) {
+      st_relaxed(&ctrl[kCtrlMoveExit], 0u);
+      st_relaxed(&ctrl[kCtrlMoveEpoch], epoch + 1u);
+      if (err_out) {
+        const uint32_t e = ld_relaxed(&ctrl[kCtrlError]);
+        if (e) or_relaxed(err_out, e);
+      }
+    }
   }
 }
 
@@ -322,9 +332,10 @@ __global__ __launch_bounds__(WG) void rz_move_pair(T* __restrict__ A, const int3
                                                     uint32_t sZ, uint32_t L, uint32_t ns,
                                                     uint32_t* __restrict__ bits, uint64_t* __restrict__ mstatus,
                                                     uint64_t* __restrict__ slots, uint32_t slot_cap,
-                                                    uint32_t* __restrict__ ctrl, int stats, int early) {
+                                                    uint32_t* __restrict__ ctrl, int stats, int early,
+                                                    uint32_t* __restrict__ err_out) {
   move_pair_body<T, WG, IPT, false>(A, R, k, sZ, L, ns, bits, mstatus, slots, slot_cap, ctrl, stats, early,
-                                    blockIdx.x, 0u);
+                                    blockIdx.x, 0u, err_out);
 }
 
 // Variant 6: the mark step in the same launch.  Workgroups [0, nm) each mark
@@ -342,7 +353,8 @@ __global__ __launch_bounds__(WG) void rz_mark_move_pair(T* __restrict__ A, const
                                                          uint64_t* __restrict__ mstatus,
                                                          uint64_t* __restrict__ slots, uint32_t slot_cap,
                                                          uint32_t* __restrict__ ctrl, int stats, uint32_t nm,
-                                                         uint32_t tile, uint32_t backoff, int sorted, int early) {
+                                                         uint32_t tile, uint32_t backoff, int sorted, int early,
+                                                         uint32_t* __restrict__ err_out) {
   if (blockIdx.x < nm) {
     const uint32_t lo = blockIdx.x * tile, hi = min(lo + tile, k);
     if (!sorted || (uint32_t)R[hi - 1] >= sZ) {  // a sorted chunk below the red zone has no marks
@@ -364,7 +376,7 @@ __global__ __launch_bounds__(WG) void rz_mark_move_pair(T* __restrict__ A, const
     return;
   }
   move_pair_body<T, WG, IPT, true>(A, R, k, sZ, L, ns, bits, mstatus, slots, slot_cap, ctrl, stats, early,
-                                   blockIdx.x - nm, backoff);
+                                   blockIdx.x - nm, backoff, err_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -515,12 +527,12 @@ static bool dispatch_move(int wg, int ipt, dim3 grid, hipStream_t s, T* A, const
 template <typename T>
 static bool dispatch_move_pair(int wg, int ipt, hipStream_t s, T* A, const int32_t* R, uint32_t k, uint32_t sZ,
                                uint32_t L, uint32_t ns, uint32_t* bits, uint64_t* mstatus, uint64_t* slots,
-                               uint32_t slot_cap, uint32_t* ctrl, int stats, int early) {
+                               uint32_t slot_cap, uint32_t* ctrl, int stats, int early, uint32_t* err_out = nullptr) {
   const size_t lds = (size_t)L * 4;
 #define RZ_MP_CASE(W, I)                                                                                     \
   if (wg == W && ipt == I) {                                                                                 \
     hipLaunchKernelGGL((rz_move_pair<T, W, I>), dim3(ns), dim3(W), lds, s, A, R, k, sZ, L, ns, bits, mstatus, \
-                       slots, slot_cap, ctrl, stats, early);                                                 \
+                       slots, slot_cap, ctrl, stats, early, err_out);                                        \
     return true;                                                                                             \
   }
   RZ_MP_CASE(256, 2) RZ_MP_CASE(256, 4) RZ_MP_CASE(256, 8) RZ_MP_CASE(256, 16)
@@ -534,7 +546,7 @@ template <typename T>
 static bool dispatch_mark_move_pair(int wg, int ipt, hipStream_t s, T* A, const int32_t* R, uint32_t k, uint32_t sZ,
                                     uint32_t L, uint32_t ns, uint32_t* bits, uint64_t* mstatus, uint64_t* slots,
                                     uint32_t slot_cap, uint32_t* ctrl, int stats, int tile_log, int sorted,
-                                    int early) {
+                                    int early, uint32_t* err_out = nullptr) {
   const uint32_t bo = kMarkWaitBackoff;
   const size_t lds = (size_t)L * 4;
   tile_log = tile_log == 0 ? kFusedMarkTileLog : (tile_log < 8 ? 8 : (tile_log > 16 ? 16 : tile_log));
@@ -543,7 +555,7 @@ static bool dispatch_mark_move_pair(int wg, int ipt, hipStream_t s, T* A, const 
 #define RZ_MMP_CASE(W, I)                                                                                      \
   if (wg == W && ipt == I) {                                                                                   \
     hipLaunchKernelGGL((rz_mark_move_pair<T, W, I>), dim3(nm + ns), dim3(W), lds, s, A, R, k, sZ, L, ns, bits, \
-                       mstatus, slots, slot_cap, ctrl, stats, nm, tile, bo, sorted, early);                    \
+                       mstatus, slots, slot_cap, ctrl, stats, nm, tile, bo, sorted, early, err_out);           \
     return true;                                                                                               \
   }
   RZ_MMP_CASE(256, 2) RZ_MMP_CASE(256, 4) RZ_MMP_CASE(256, 8) RZ_MMP_CASE(256, 16)
@@ -685,6 +697,71 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
   hipLaunchKernelGGL(rz_pair<T>, g3, dim3(256), 0, s, A, Rd, (uint32_t)k, sZ, L, ns, counts, ctrl, stats);
   RZ_HIP(hipGetLastError());
   return prof_mark(ctx, flags, 3, s);
+}
+
+// ---------------------------------------------------------------------------
+// the fused forms on scratch the caller provides (rz_remove_noalloc)
+// ---------------------------------------------------------------------------
+// Clears what a fused call needs at zero: the tail bitmap words of this k, the
+// error word and the two exit counters.  A kernel, not a memset, so that the
+// whole call can be captured in a graph.
+__global__ __launch_bounds__(256) void rz_static_prologue(uint32_t* __restrict__ bits, uint32_t nwords,
+                                                          uint32_t* __restrict__ ctrl) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  for (uint32_t w = i; w < nwords; w += gridDim.x * 256u) bits[w] = 0u;
+  if (i == 0) {
+    ctrl[kCtrlError] = 0u;
+    ctrl[kCtrlMoveExit] = 0u;
+    ctrl[kCtrlMarkDone] = 0u;
+  }
+}
+
+template <typename T>
+static rz_status fused_static_impl(T* A, int64_t n, const int32_t* R, int64_t k, uint32_t flags,
+                                   const rz_tile_config& cfg, const FusedScratch& fs, hipStream_t s) {
+  const uint32_t sZ = (uint32_t)(n - k);
+  uint32_t L, ns;
+  slice_shape(k, cfg, kMaxSlices, &L, &ns);
+  if ((size_t)L * 4 + 4096 > 65536 || cfg.wg_size * cfg.items > (int)L) return RZ_EINVAL;
+  const uint32_t nwords = (uint32_t)((k + 31) / 32);
+  const uint32_t g0 = (nwords + 255) / 256;
+  hipLaunchKernelGGL(rz_static_prologue, dim3(g0 < 1024 ? (g0 ? g0 : 1) : 1024), dim3(256), 0, s, fs.bits, nwords,
+                     fs.ctrl);
+  RZ_HIP(hipGetLastError());
+  const int early = (flags & RZ_ANY_PAIRING) ? 1 : 0;
+  const uint32_t slot_cap = (uint32_t)(k / 2 + 1);
+  if (cfg.variant == 6) {
+    if (!dispatch_mark_move_pair<T>(cfg.wg_size, cfg.items, s, A, R, (uint32_t)k, sZ, L, ns, fs.bits, fs.mstatus,
+                                    fs.slots, slot_cap, fs.ctrl, 0, cfg.slice_steps, (flags & RZ_SORTED) ? 1 : 0,
+                                    early, fs.err_out))
+      return RZ_EINVAL;
+  } else {
+    if (flags & RZ_SORTED)
+      hipLaunchKernelGGL(rz_mark_tail_sorted, dim3((uint32_t)((k + kSortedChunk - 1) / kSortedChunk)), dim3(256), 0,
+                         s, R, (uint32_t)k, sZ, fs.bits);
+    else {
+      const int64_t g1w = (k + 256 * 4 - 1) / (256 * 4);
+      hipLaunchKernelGGL(rz_mark_tail<false>, dim3((uint32_t)(g1w < 2048 ? g1w : 2048)), dim3(256), 0, s, R,
+                         (uint32_t)k, sZ, fs.bits, (uint64_t*)nullptr);
+    }
+    RZ_HIP(hipGetLastError());
+    if (!dispatch_move_pair<T>(cfg.wg_size, cfg.items, s, A, R, (uint32_t)k, sZ, L, ns, fs.bits, fs.mstatus,
+                               fs.slots, slot_cap, fs.ctrl, 0, early, fs.err_out))
+      return RZ_EINVAL;
+  }
+  RZ_HIP(hipGetLastError());
+  return RZ_OK;
+}
+
+rz_status launch_fused_static(void* data, int64_t n, uint32_t esz, const int32_t* R, int64_t k, uint32_t flags,
+                              const rz_tile_config& cfg, const FusedScratch& fs, hipStream_t s) {
+  if (cfg.variant != 4 && cfg.variant != 6) return RZ_EINVAL;
+  switch (esz) {
+    case 4: return fused_static_impl<uint32_t>((uint32_t*)data, n, R, k, flags, cfg, fs, s);
+    case 8: return fused_static_impl<uint64_t>((uint64_t*)data, n, R, k, flags, cfg, fs, s);
+    case 16: return fused_static_impl<uint4>((uint4*)data, n, R, k, flags, cfg, fs, s);
+    default: return RZ_EINVAL;
+  }
 }
 
 rz_status launch_plain(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, int32_t* R, int64_t k, uint32_t flags,

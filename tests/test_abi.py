@@ -44,7 +44,7 @@ def test_status_strings():
     from redzone_amd import _native
 
     lib = _native.load()
-    for s in range(9):
+    for s in range(10):
         assert lib.rz_status_string(s) and lib.rz_status_string(s) != b"unknown status"
     assert lib.rz_status_string(0) == b"ok"
 
