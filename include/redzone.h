@@ -192,6 +192,17 @@ rz_status rz_last_stats(rz_ctx* ctx, rz_stats* out);
  * move (or the whole cooperative kernel / the partition pass), ms[2] orphan
  * pairing, ms[3] first kernel start to last kernel end.  Waits for the call. */
 rz_status rz_last_profile(rz_ctx* ctx, float ms[4]);
+/* Stage timeline (us) of the last fused-form call (variants 4 and 6) made with
+ * RZ_PROFILE: every workgroup stamps the wall clock at its stage boundaries;
+ * times are from the first workgroup's start.  The counterpart of the
+ * reference's per-phase clock64() stamps (redzone.hxx:629, 708-712, 847-851,
+ * 936-939).  out[0] last mark workgroup done, out[1] ready words raised,
+ * out[2] / out[3] first / last slice started, then the last slice to finish
+ * each stage: out[4] ready wait, out[5] moves, out[6] look-back, out[7]
+ * orphan exchange, out[8] exit; out[9..13] the mean time a slice spends in
+ * each of those five stages; out[14] mark workgroups, out[15] slices.
+ * Synchronises the device.  RZ_EINVAL if the last call had no stamps. */
+rz_status rz_last_stages(rz_ctx* ctx, float out[16]);
 
 /* ---- O(n) stable partition ---------------------------------------------- */
 /* Replaces the O(n) comparators: remove_phase1 (DataStruct.hxx:11-18) followed

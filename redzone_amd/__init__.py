@@ -220,6 +220,17 @@ class Context:
         self._call("rz_last_profile", self._h, ms)
         return list(ms)
 
+    STAGE_NAMES = ("marks_done", "ready_raised", "first_slice_start", "last_slice_start", "ready_wait_end",
+                   "moves_end", "lookback_end", "exchange_end", "exit", "mean_ready_wait", "mean_moves",
+                   "mean_lookback", "mean_exchange", "mean_exit_tail", "mark_groups", "slices")
+
+    def last_stages(self):
+        """Stage timeline (us from the first workgroup's start) of the last
+        fused-form call made with RZ_PROFILE (see rz_last_stages)."""
+        out = (ctypes.c_float * 16)()
+        self._call("rz_last_stages", self._h, out)
+        return dict(zip(self.STAGE_NAMES, [round(v, 3) for v in out]))
+
     def last_stats(self):
         out = Stats()
         self._call("rz_last_stats", self._h, ctypes.byref(out))

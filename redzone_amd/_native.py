@@ -142,6 +142,7 @@ SIGNATURES = [
     ("rz_check_errors", _i32, [_vp, _vp, ctypes.POINTER(_u32)]),
     ("rz_debug_set_waits", _i32, [_vp, _u32, _u32]),
     ("rz_remove_noalloc", _i32, [_vp, _i64, _u32, _vp, _i64, _u32, _vp, _vp]),
+    ("rz_last_stages", _i32, [_vp, ctypes.POINTER(ctypes.c_float)]),
 ]
 
 TUNED_TABLE = os.path.join(_HERE, "tuned_gfx950.txt")

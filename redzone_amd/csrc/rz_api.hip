@@ -1,6 +1,9 @@
 // rz_api.hip -- C ABI entry points, context and scratch management.
 #include <string.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "rz_ctx.hpp"
 #include "rz_device.hpp"
 
@@ -175,7 +178,7 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
   for (rz_batch* b : ctx->batches) batch_detach(b);
   ctx->batches.clear();
   if (ctx->last_batch) rz_batch_destroy(ctx->last_batch);
-  DeviceBuffer* bufs[] = {&ctx->tail_bits, &ctx->list_bits, &ctx->counts, &ctx->orphans,
+  DeviceBuffer* bufs[] = {&ctx->tail_bits, &ctx->list_bits, &ctx->counts, &ctx->orphans, &ctx->stamps,
                           &ctx->status,    &ctx->ctrl,      &ctx->queue,  &ctx->mstatus, &ctx->slots,
                           &ctx->dense_cnt, &ctx->dense_small, &ctx->dense_ent, &ctx->dense_fine,
                           &ctx->dense_src, &ctx->dispatch, &ctx->gstatus, &ctx->ploaded, &ctx->pgcnt,
@@ -311,6 +314,59 @@ rz_status rz_last_profile(rz_ctx* ctx, float ms[4]) {
   RZ_HIP(hipEventElapsedTime(&ms[1], ctx->ev[1], ctx->ev[2]));
   RZ_HIP(hipEventElapsedTime(&ms[2], ctx->ev[2], ctx->ev[3]));
   RZ_HIP(hipEventElapsedTime(&ms[3], ctx->ev[0], ctx->ev[3]));
+  return RZ_OK;
+}
+
+rz_status rz_last_stages(rz_ctx* ctx, float out[16]) {
+  if (!ctx || !out) return RZ_EINVAL;
+  memset(out, 0, 16 * sizeof(float));
+  if (!ctx->stamp_groups || !ctx->stamps.ptr) return RZ_EINVAL;
+  DeviceGuard g(ctx->device);
+  const uint32_t groups = ctx->stamp_groups, nm = ctx->stamp_marks;
+  std::vector<uint64_t> h((size_t)groups * kStampsPerGroup);
+  RZ_HIP(hipDeviceSynchronize());
+  RZ_HIP(hipMemcpy(h.data(), ctx->stamps.ptr, h.size() * 8, hipMemcpyDeviceToHost));
+  int khz = 0;
+  RZ_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+  if (khz <= 0) return RZ_EHIP;
+  const double us_per_tick = 1e3 / (double)khz;
+  auto row = [&](uint32_t gidx, int i) { return h[(size_t)gidx * kStampsPerGroup + i]; };
+  uint64_t t0 = ~0ull;
+  for (uint32_t gi = 0; gi < groups; ++gi)
+    if (row(gi, kStampStart) && row(gi, kStampStart) < t0) t0 = row(gi, kStampStart);
+  if (t0 == ~0ull) return RZ_EINVAL;
+  auto rel = [&](uint64_t t) { return t ? (float)((double)(t - t0) * us_per_tick) : 0.f; };
+  uint64_t marked = 0, raised = 0;
+  for (uint32_t gi = 0; gi < nm; ++gi) {
+    marked = std::max(marked, row(gi, kStampMarked));
+    raised = std::max(raised, row(gi, kStampRaised));
+  }
+  uint64_t first = ~0ull, last = 0, mx[6] = {0, 0, 0, 0, 0, 0};
+  double sum[6] = {0, 0, 0, 0, 0, 0};
+  const uint32_t ns = groups - nm;
+  for (uint32_t gi = nm; gi < groups; ++gi) {
+    const uint64_t st0 = row(gi, kStampStart);
+    first = std::min(first, st0);
+    last = std::max(last, st0);
+    uint64_t prev = st0;
+    for (int i = kStampReady; i <= kStampExit; ++i) {
+      const uint64_t t = row(gi, i);
+      if (!t) continue;  // no ready wait in the two-kernel form
+      mx[i] = std::max(mx[i], t);
+      sum[i] += (double)(t - prev) * us_per_tick;
+      prev = t;
+    }
+  }
+  out[0] = rel(marked);
+  out[1] = rel(raised);
+  out[2] = rel(first);
+  out[3] = rel(last);
+  for (int i = kStampReady; i <= kStampExit; ++i) {
+    out[3 + i] = rel(mx[i]);                            // out[4..8]: last slice to reach the stage
+    out[8 + i] = ns ? (float)(sum[i] / (double)ns) : 0.f;  // out[9..13]: mean time a slice spends in it
+  }
+  out[14] = (float)nm;
+  out[15] = (float)ns;
   return RZ_OK;
 }
 

@@ -47,6 +47,20 @@ __device__ __forceinline__ bool stalled(const uint32_t* ctrl) {
   return __hip_atomic_load(&ctrl[kCtrlStall], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
 }
 
+// RZ_PROFILE stage stamps of the fused forms: one row of wall-clock values per
+// workgroup (mark workgroups of the one-launch form first, then the slices)
+enum StampIndex : int {
+  kStampStart = 0,     // the workgroup started
+  kStampReady = 1,     // slice: the marks of this launch are in (one-launch form)
+  kStampMoves = 2,     // slice: every step's moves issued, its counts published
+  kStampLookback = 3,  // slice: the look-back has the earlier slices' orphan counts
+  kStampExchange = 4,  // slice: every orphan of the slice went through its exchange word
+  kStampExit = 5,      // slice: left (after clearing its bitmap words and counting its exit)
+  kStampMarked = 1,    // mark workgroup: its marks are done
+  kStampRaised = 2,    // mark workgroup: (the last one) raised the ready words
+  kStampsPerGroup = 8
+};
+
 // internal flag: the tail bitmap was set by the caller (rz_remove_premarked)
 constexpr uint32_t kFlagPremarked = 0x80000000u;
 
@@ -95,6 +109,9 @@ struct rz_ctx {
   rz_tile_config table[3][33];    // [element size 4/8/16][floor(log2 k)]
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // RZ_PROFILE marks
   bool ev_recorded = false;
+  rz::DeviceBuffer stamps;     // RZ_PROFILE: stage stamps of the last fused call (StampIndex rows)
+  uint32_t stamp_groups = 0;   // rows written by that call (0: the last call had none)
+  uint32_t stamp_marks = 0;    // of which mark workgroups (one-launch form)
   std::mutex mu;
 };
 

@@ -126,7 +126,7 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
                                                uint64_t* __restrict__ mstatus, uint64_t* __restrict__ slots,
                                                uint32_t slot_cap, uint32_t* __restrict__ ctrl, int stats,
                                                int early, uint32_t slice, uint32_t backoff,
-                                               uint32_t* __restrict__ err_out) {
+                                               uint32_t* __restrict__ err_out, uint64_t* __restrict__ stamps) {
   constexpr int NW = WG / kWave;
   constexpr int E = MoveShared<WG, IPT>::E;
   constexpr int PER = (E + kWave - 1) / kWave;
@@ -137,6 +137,9 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
   const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
   const uint32_t a = slice * L, b = min(a + L, k);
   T* __restrict__ Z = A + sZ;
+  // RZ_PROFILE: thread 0 stamps the wall clock at each stage boundary
+  uint64_t* const my_stamps = stamps ? stamps + (size_t)blockIdx.x * kStampsPerGroup : nullptr;
+  if (my_stamps && tid == 0) my_stamps[kStampStart] = wall_clock64();
   const uint32_t epoch = ld_relaxed(&ctrl[kCtrlMoveEpoch]);
   uint64_t* __restrict__ st = mstatus + (size_t)(epoch & 1u) * kMaxSlices;
   {
@@ -168,7 +171,10 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
     load_rz(base, d, z, wv, !COH);
     if constexpr (COH) {
       // every mark of this launch is in: this XCD's copy of the ready word
-      if (base == a) wait_value_backoff(&ctrl[kCtrlMarkReady + 32 * (blockIdx.x & 7u)], epoch + 1u, ctrl, backoff);
+      if (base == a) {
+        wait_value_backoff(&ctrl[kCtrlMarkReady + 32 * (blockIdx.x & 7u)], epoch + 1u, ctrl, backoff);
+        if (my_stamps && tid == 0) my_stamps[kStampReady] = wall_clock64();
+      }
 #pragma unroll
       for (int j = 0; j < IPT; ++j) {
         const uint32_t p = base + j * WG + tid;
@@ -235,6 +241,7 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
     __syncthreads();
   }
 
+  if (my_stamps && tid == 0) my_stamps[kStampMoves] = wall_clock64();
   // Early pairing (RZ_ANY_PAIRING): the slice's first m dest orphans take its
   // first m source orphans here, and only the rest go to the global exchange.
   const uint32_t m = early ? min(carry_d, carry_s) : 0u;
@@ -287,6 +294,7 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
     }
   }
   __syncthreads();
+  if (my_stamps && tid == 0) my_stamps[kStampLookback] = wall_clock64();
 
   // meet the partners
   const uint64_t excl = s_excl;
@@ -308,6 +316,10 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
     }
     st_relaxed(&slots[t], 0ull);
   }
+  if (my_stamps) {
+    __syncthreads();
+    if (tid == 0) my_stamps[kStampExchange] = wall_clock64();
+  }
   // leave the bitmap zero for the next call (slices are word aligned)
   for (uint32_t w = (a >> 5) + tid; w < ((b + 31u) >> 5); w += WG) bits[w] = 0u;
   // the last workgroup to leave (all of them have read the epoch by then)
@@ -324,6 +336,7 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
         if (e) or_relaxed(err_out, e);
       }
     }
+    if (my_stamps) my_stamps[kStampExit] = wall_clock64();
   }
 }
 
@@ -333,9 +346,9 @@ __global__ __launch_bounds__(WG) void rz_move_pair(T* __restrict__ A, const int3
                                                     uint32_t* __restrict__ bits, uint64_t* __restrict__ mstatus,
                                                     uint64_t* __restrict__ slots, uint32_t slot_cap,
                                                     uint32_t* __restrict__ ctrl, int stats, int early,
-                                                    uint32_t* __restrict__ err_out) {
+                                                    uint32_t* __restrict__ err_out, uint64_t* __restrict__ stamps) {
   move_pair_body<T, WG, IPT, false>(A, R, k, sZ, L, ns, bits, mstatus, slots, slot_cap, ctrl, stats, early,
-                                    blockIdx.x, 0u, err_out);
+                                    blockIdx.x, 0u, err_out, stamps);
 }
 
 // Variant 6: the mark step in the same launch.  Workgroups [0, nm) each mark
@@ -354,8 +367,11 @@ __global__ __launch_bounds__(WG) void rz_mark_move_pair(T* __restrict__ A, const
                                                          uint64_t* __restrict__ slots, uint32_t slot_cap,
                                                          uint32_t* __restrict__ ctrl, int stats, uint32_t nm,
                                                          uint32_t tile, uint32_t backoff, int sorted, int early,
-                                                         uint32_t* __restrict__ err_out) {
+                                                         uint32_t* __restrict__ err_out,
+                                                         uint64_t* __restrict__ stamps) {
   if (blockIdx.x < nm) {
+    uint64_t* const my_stamps = stamps ? stamps + (size_t)blockIdx.x * kStampsPerGroup : nullptr;
+    if (my_stamps && threadIdx.x == 0) my_stamps[kStampStart] = wall_clock64();
     const uint32_t lo = blockIdx.x * tile, hi = min(lo + tile, k);
     if (!sorted || (uint32_t)R[hi - 1] >= sZ) {  // a sorted chunk below the red zone has no marks
       // a whole tile's loads in flight at once (16 per thread for 4,096 removals and 256 threads)
@@ -368,15 +384,17 @@ __global__ __launch_bounds__(WG) void rz_mark_move_pair(T* __restrict__ A, const
     // ready words (one per XCD, so that the waiting slices poll 8 lines)
     wait_all_memory();
     __syncthreads();
+    if (my_stamps && threadIdx.x == 0) my_stamps[kStampMarked] = wall_clock64();
     if (threadIdx.x == 0 && add_relaxed(&ctrl[kCtrlMarkDone], 1u) == nm - 1u) {
       st_relaxed(&ctrl[kCtrlMarkDone], 0u);
       const uint32_t ready = ld_relaxed(&ctrl[kCtrlMoveEpoch]) + 1u;  // no slice can have finished yet
       for (int x = 0; x < 8; ++x) st_relaxed(&ctrl[kCtrlMarkReady + 32 * x], ready);
+      if (my_stamps) my_stamps[kStampRaised] = wall_clock64();
     }
     return;
   }
   move_pair_body<T, WG, IPT, true>(A, R, k, sZ, L, ns, bits, mstatus, slots, slot_cap, ctrl, stats, early,
-                                   blockIdx.x - nm, backoff, err_out);
+                                   blockIdx.x - nm, backoff, err_out, stamps);
 }
 
 // ---------------------------------------------------------------------------
@@ -527,12 +545,13 @@ static bool dispatch_move(int wg, int ipt, dim3 grid, hipStream_t s, T* A, const
 template <typename T>
 static bool dispatch_move_pair(int wg, int ipt, hipStream_t s, T* A, const int32_t* R, uint32_t k, uint32_t sZ,
                                uint32_t L, uint32_t ns, uint32_t* bits, uint64_t* mstatus, uint64_t* slots,
-                               uint32_t slot_cap, uint32_t* ctrl, int stats, int early, uint32_t* err_out = nullptr) {
+                               uint32_t slot_cap, uint32_t* ctrl, int stats, int early, uint32_t* err_out = nullptr,
+                               uint64_t* stamps = nullptr) {
   const size_t lds = (size_t)L * 4;
 #define RZ_MP_CASE(W, I)                                                                                     \
   if (wg == W && ipt == I) {                                                                                 \
     hipLaunchKernelGGL((rz_move_pair<T, W, I>), dim3(ns), dim3(W), lds, s, A, R, k, sZ, L, ns, bits, mstatus, \
-                       slots, slot_cap, ctrl, stats, early, err_out);                                        \
+                       slots, slot_cap, ctrl, stats, early, err_out, stamps);                                \
     return true;                                                                                             \
   }
   RZ_MP_CASE(256, 2) RZ_MP_CASE(256, 4) RZ_MP_CASE(256, 8) RZ_MP_CASE(256, 16)
@@ -541,21 +560,30 @@ static bool dispatch_move_pair(int wg, int ipt, hipStream_t s, T* A, const int32
   return false;
 }
 
+static int clamp_tile_log(int tile_log) {
+  return tile_log == 0 ? kFusedMarkTileLog : (tile_log < 8 ? 8 : (tile_log > 16 ? 16 : tile_log));
+}
+// mark workgroups of the one-launch form for k removals
+static uint32_t mark_groups(int64_t k, int tile_log) {
+  const uint32_t tile = 1u << clamp_tile_log(tile_log);
+  return (uint32_t)((k + tile - 1) / tile);
+}
+
 // Variant 6: nm mark workgroups, then the ns move-and-pair slices, one grid.
 template <typename T>
 static bool dispatch_mark_move_pair(int wg, int ipt, hipStream_t s, T* A, const int32_t* R, uint32_t k, uint32_t sZ,
                                     uint32_t L, uint32_t ns, uint32_t* bits, uint64_t* mstatus, uint64_t* slots,
                                     uint32_t slot_cap, uint32_t* ctrl, int stats, int tile_log, int sorted,
-                                    int early, uint32_t* err_out = nullptr) {
+                                    int early, uint32_t* err_out = nullptr, uint64_t* stamps = nullptr) {
   const uint32_t bo = kMarkWaitBackoff;
   const size_t lds = (size_t)L * 4;
-  tile_log = tile_log == 0 ? kFusedMarkTileLog : (tile_log < 8 ? 8 : (tile_log > 16 ? 16 : tile_log));
+  tile_log = clamp_tile_log(tile_log);
   const uint32_t tile = 1u << tile_log;
   const uint32_t nm = (k + tile - 1) / tile;
 #define RZ_MMP_CASE(W, I)                                                                                      \
   if (wg == W && ipt == I) {                                                                                   \
     hipLaunchKernelGGL((rz_mark_move_pair<T, W, I>), dim3(nm + ns), dim3(W), lds, s, A, R, k, sZ, L, ns, bits, \
-                       mstatus, slots, slot_cap, ctrl, stats, nm, tile, bo, sorted, early, err_out);           \
+                       mstatus, slots, slot_cap, ctrl, stats, nm, tile, bo, sorted, early, err_out, stamps);   \
     return true;                                                                                               \
   }
   RZ_MMP_CASE(256, 2) RZ_MMP_CASE(256, 4) RZ_MMP_CASE(256, 8) RZ_MMP_CASE(256, 16)
@@ -615,6 +643,7 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
   uint32_t L, ns;
   slice_shape(k, cfg, kMaxSlices, &L, &ns);
   rz_status st;
+  ctx->stamp_groups = 0;  // stage stamps only from a fused call with RZ_PROFILE
   if (!ext_bits && (st = ensure(ctx, ctx->tail_bits, (size_t)((k + 63) / 64) * 8, s)) != RZ_OK) return st;
   if ((st = ensure(ctx, ctx->counts, (size_t)ns * 8, s)) != RZ_OK) return st;
   int32_t* Rd = R;
@@ -670,11 +699,20 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
     if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;
     return prof_mark(ctx, flags, 3, s);
   }
+  uint64_t* stamps = nullptr;
+  if (fused && (flags & RZ_PROFILE)) {
+    const uint32_t groups = ns + (one_launch ? mark_groups(k, cfg.slice_steps) : 0u);
+    if ((st = ensure(ctx, ctx->stamps, (size_t)groups * kStampsPerGroup * 8, s)) != RZ_OK) return st;
+    stamps = (uint64_t*)ctx->stamps.ptr;
+    if ((st = fill_u32((uint32_t*)stamps, 0u, (size_t)groups * kStampsPerGroup * 2, s)) != RZ_OK) return st;
+    ctx->stamp_groups = groups;
+    ctx->stamp_marks = one_launch ? mark_groups(k, cfg.slice_steps) : 0u;
+  }
   if (one_launch) {
     if (!dispatch_mark_move_pair<T>(cfg.wg_size, cfg.items, s, A, R, (uint32_t)k, sZ, L, ns, bits,
                                     (uint64_t*)ctx->mstatus.ptr, (uint64_t*)ctx->slots.ptr, (uint32_t)(k / 2 + 1),
                                     ctrl, stats, cfg.slice_steps, (flags & RZ_SORTED) ? 1 : 0,
-                                    (flags & RZ_ANY_PAIRING) ? 1 : 0))
+                                    (flags & RZ_ANY_PAIRING) ? 1 : 0, nullptr, stamps))
       return RZ_EINVAL;
     RZ_HIP(hipGetLastError());
     if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;
@@ -683,7 +721,7 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
   if (fused) {
     if (!dispatch_move_pair<T>(cfg.wg_size, cfg.items, s, A, R, (uint32_t)k, sZ, L, ns, bits,
                                (uint64_t*)ctx->mstatus.ptr, (uint64_t*)ctx->slots.ptr, (uint32_t)(k / 2 + 1), ctrl,
-                               stats, (flags & RZ_ANY_PAIRING) ? 1 : 0))
+                               stats, (flags & RZ_ANY_PAIRING) ? 1 : 0, nullptr, stamps))
       return RZ_EINVAL;
     RZ_HIP(hipGetLastError());
     if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;

@@ -447,3 +447,27 @@ def test_any_pairing(ctx, oracle, esz, variant):
         assert s.orphan_pairs <= canonical_pairs
         if ms > 1 and canonical_pairs > 1000:
             assert s.orphan_pairs < canonical_pairs  # slices with both kinds paired some themselves
+
+
+@pytest.mark.parametrize("variant", [4, 6])
+def test_stage_stamps(ctx, oracle, variant):
+    """RZ_PROFILE on the fused forms: every workgroup stamps its stage
+    boundaries; the timeline has the launch's workgroup counts and is ordered
+    stage by stage; a call of another form leaves no stamps to read."""
+    n = 1 << 22
+    k = n // 100
+    A = W.unique_u32_payload(n, seed=95)
+    R = W.uniform_removals(n, k, seed=96)
+    cfg = rz.TileConfig(256, 4, 512, variant, 12 if variant == 6 else 0, 0)
+    announce("stage stamps", variant)
+    gpu_remove_parity(ctx, oracle, A, R, config=cfg, flags=rz.RZ_PROFILE)
+    st = ctx.last_stages()
+    assert st["slices"] == (k + 1023) // 1024  # one step of 256 x 4 pairs per slice at this k
+    assert st["mark_groups"] == ((k + 4095) // 4096 if variant == 6 else 0)
+    assert 0 <= st["first_slice_start"] <= st["last_slice_start"]
+    assert 0 < st["moves_end"] <= st["lookback_end"] <= st["exchange_end"] <= st["exit"] < 10_000
+    if variant == 6:
+        assert 0 < st["marks_done"] <= st["ready_raised"] <= st["ready_wait_end"] <= st["moves_end"]
+    gpu_remove_parity(ctx, oracle, A, R, config=rz.TileConfig(256, 4, 64, 0, 0, 0), flags=rz.RZ_PROFILE)
+    with pytest.raises(rz.RedzoneError):
+        ctx.last_stages()
