@@ -16,9 +16,11 @@ restatement of Redzone_CPU timed on this host's cores (n/t, same workload).
 Everything else goes to stderr.
 """
 import argparse
+import hashlib
 import json
 import os
 import statistics
+import subprocess
 import sys
 import time
 
@@ -45,7 +47,7 @@ def parse():
     p.add_argument("--warm", action="store_true", help="skip the cache scrub before each call")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-extra", action="store_true", help="skip the comparator measurements")
-    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--cpu-seconds", type=float, default=8.0, help="CPU time budget of each CPU-baseline measurement")
     p.add_argument("--variant", type=int, default=None,
                    help="0 three kernels, 1 cooperative, 2 persistent queue, 3 ordered, 4 fused, 5 dense")
     p.add_argument("--wg", type=int, default=256)
@@ -110,6 +112,67 @@ PATH_OF_VARIANT = {0: "rz_remove (mark, move, pair kernels)", 1: "rz_remove (one
                    2: "rz_remove (persistent queue kernel)", 3: "rz_remove (one ordered launch)",
                    4: "rz_remove (mark kernel, then move with in-kernel orphan pairing)",
                    5: "rz_remove (dense form: bucketed marks and region-grouped writes)"}
+
+
+def library_sha16():
+    """First 16 hex digits of the SHA-256 of the loaded librz_amd.so (the build
+    is deterministic, so the same sources give the same bytes)."""
+    from redzone_amd import _native
+
+    with open(_native.LIB_PATH, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
+
+
+def load_traffic(kernel, shape):
+    """HBM bytes per launch of `kernel` from the committed counter record, or
+    (None, reason) when the record was measured on another build or launch
+    shape than the one this run loaded."""
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{kernel}_u32_n2e28_k1pct.json")
+    if not os.path.exists(tpath):
+        return None, "no counter record for this kernel"
+    with open(tpath) as f:
+        rec = json.load(f)
+    sha = library_sha16()
+    if rec.get("library_sha256_16") != sha:
+        return None, f"counter record is of library {rec.get('library_sha256_16')}, this run loaded {sha}"
+    if tuple(rec.get("launch_shape", ())) != tuple(shape):
+        return None, f"counter record is of launch shape {rec.get('launch_shape')}, this run used {list(shape)}"
+    return rec.get("hbm_bytes_per_launch"), None
+
+
+def scatter_floor(torch, A, A0, R0, n, k, restore, scrub, steps, time_fn):
+    """The same hole writes issued by a plain one-pass scatter (torch's
+    index_copy_: read an index and a value, write the value at the index), with
+    the holes in the removal list's order and sorted by address, cold like the
+    timed calls.  Every correct removal writes these holes; their time is the
+    floor of the scattered part of the call."""
+    sz = n - k
+    holes = R0[R0 < sz].long()
+    vals = A0[sz: sz + holes.numel()].clone()
+    ordered = torch.sort(holes).values
+    t_rand, _, _ = time_fn(torch, None, restore, lambda: A.index_copy_(0, holes, vals), steps, 2, scrub, False)
+    t_sort, _, _ = time_fn(torch, None, restore, lambda: A.index_copy_(0, ordered, vals), steps, 2, scrub, False)
+    return {"holes": int(holes.numel()), "random_order_us": round(statistics.median(t_rand) * 1e3, 2),
+            "address_order_us": round(statistics.median(t_sort) * 1e3, 2),
+            "kind": "torch index_copy_ of the hole positions, cold cache, event-timed"}
+
+
+def stage_timeline(ctx, restore, scrub, call, reps=5):
+    """Mean stage timeline (us) of `reps` cold calls made with RZ_PROFILE
+    (fused forms only; None otherwise)."""
+    import redzone_amd as rz
+
+    rows = []
+    for _ in range(reps):
+        restore()
+        if scrub is not None:
+            scrub()
+        call(rz.RZ_PROFILE)
+        try:
+            rows.append(ctx.last_stages())
+        except rz.RedzoneError:
+            return None
+    return {key: round(statistics.mean(r[key] for r in rows), 2) for key in rows[0]}
 
 
 class Scrubber:
@@ -232,42 +295,73 @@ def bench_mixed(torch, ctx, args):
     return out
 
 
+def host_cpus():
+    """The host's CPUs as this process sees them: the count the reference's
+    hardware_concurrency() reports (all online CPUs), the affinity mask, the
+    cgroup CPU quota and the model from lscpu."""
+    info = {"os_cpu_count": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        info["cgroup_cpus"] = None if quota == "max" else round(float(quota) / float(period), 2)
+    except (OSError, ValueError):
+        info["cgroup_cpus"] = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            key, _, val = line.partition(":")
+            if key.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info["lscpu_" + key.strip().lower().replace("(s)", "s").replace(" ", "_")] = val.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
 def cpu_baseline(A0_h, R0_h, n, k, seconds):
-    """The oracle's Redzone_CPU restatement on this host (O(k) restore between reps)."""
+    """The oracle's Redzone_CPU restatement on this host (O(k) restore between
+    reps), with T = nproc - 1 threads (the reference's hardware_concurrency() - 1,
+    Redzone_CPU.hxx:153) and with T = nproc; `value` is the T = nproc - 1 figure."""
     import numpy as np
 
     from oracle import pyoracle
 
-    cores = max(1, min(16, os.cpu_count() or 1) - 1)  # the reference uses hardware_concurrency() - 1
-    A = A0_h.copy()
-    R = R0_h.copy()
+    host = host_cpus()
+    nproc = host["os_cpu_count"] or 1
     sz = n - k
-    touched = np.concatenate([R0_h[R0_h < sz].astype(np.int64)])
-    ts = []
-    t_start = time.perf_counter()
-    while len(ts) < 20 and (time.perf_counter() - t_start) < seconds:
-        A[touched] = A0_h[touched]
-        A[sz:] = A0_h[sz:]
-        R[:] = R0_h
-        st, t = pyoracle.redzone_cpu(A, R, threads=cores)
-        assert st == 0
-        ts.append(t[0] / 1e9)
-    med = statistics.median(ts)
-    out = {"value": round(n / med / 1e9, 3), "unit": "Gitems/s", "cores": cores, "kind": "port",
-           "sample": f"full workload (n=2^{n.bit_length() - 1}, k={k}), {len(ts)} reps, median "
-                     f"{med * 1e3:.2f} ms, Redzone_CPU restatement (OpenMP, {cores} threads)",
-           "ms_median": round(med * 1e3, 3)}
+    touched = R0_h[R0_h < sz].astype(np.int64)
+    by_threads = {}
+    A = None
+    for threads in sorted({max(1, nproc - 1), nproc}):
+        A = A0_h.copy()
+        R = R0_h.copy()
+        ts = []
+        t_start = time.perf_counter()
+        while len(ts) < 20 and (time.perf_counter() - t_start) < seconds:
+            A[touched] = A0_h[touched]
+            A[sz:] = A0_h[sz:]
+            R[:] = R0_h
+            st, t = pyoracle.redzone_cpu(A, R, threads=threads)
+            assert st == 0
+            ts.append(t[0] / 1e9)
+        med = statistics.median(ts)
+        by_threads[threads] = {"ms_median": round(med * 1e3, 3), "Gitems_s": round(n / med / 1e9, 3), "reps": len(ts)}
+    t_ref = max(1, nproc - 1)
+    ref = by_threads[t_ref]
+    out = {"value": ref["Gitems_s"], "unit": "Gitems/s", "cores": t_ref, "kind": "port",
+           "sample": f"full workload (n=2^{n.bit_length() - 1}, k={k}), {ref['reps']} reps, median "
+                     f"{ref['ms_median']:.2f} ms, Redzone_CPU restatement (OpenMP, {t_ref} threads = nproc - 1 as "
+                     f"Redzone_CPU.hxx:153)",
+           "ms_median": ref["ms_median"], "by_threads": by_threads, "host": host}
     # the O(n) host comparator (std_parallel_remove.hxx:13-36 restated): full restore between reps
     B = A0_h.copy()
     ts2 = []
     t_start = time.perf_counter()
     while len(ts2) < 5 and (time.perf_counter() - t_start) < seconds:
         B[:] = A0_h
-        st, t = pyoracle.remove_stable(B, R0_h, threads=cores)
+        st, t = pyoracle.remove_stable(B, R0_h, threads=t_ref)
         assert st == 0
         ts2.append(t / 1e9)
     med2 = statistics.median(ts2)
-    out["std_parallel_remove"] = {"value": round(n / med2 / 1e9, 3), "unit": "Gitems/s", "cores": cores,
+    out["std_parallel_remove"] = {"value": round(n / med2 / 1e9, 3), "unit": "Gitems/s", "cores": t_ref,
                                   "kind": "port", "ms_median": round(med2 * 1e3, 3),
                                   "sample": f"same workload, {len(ts2)} reps, mark + stable remove (OpenMP)"}
     return out, A
@@ -337,15 +431,24 @@ def main():
     got = A.cpu().numpy()
     bit_exact = bool(np.array_equal(got[: n - k], want[: n - k]))
     log(f"bit-exact against the oracle: {bit_exact}")
+    try:  # every device-side check of the calls so far (orphan totals, bounded waits)
+        device_errors = ctx.check_errors()
+    except rz.RedzoneError as e:
+        device_errors = e.error_flags
 
     B_kernel = B if kernel_pairs else B_move
     achieved = B_kernel / (move_ms * 1e-3) / 1e9
     op_gbs = B / (t_med * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{kernel}_u32_n2e28_k1pct.json")
-    if os.path.exists(tpath) and args.log2n == 28 and args.frac == 0.01 and s == 4 and args.pattern == "uniform":
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    traffic, traffic_note = None, "counters are recorded for the default workload only"
+    if args.log2n == 28 and args.frac == 0.01 and s == 4 and args.pattern == "uniform" and cfg is None:
+        traffic, traffic_note = load_traffic(kernel, used.as_tuple())
+    # the floor of the scattered writes: the same holes written by a plain scatter
+    floor = scatter_floor(torch, A, A0, R0, n, k, restore, scrub, max(5, args.steps // 2), time_calls)
+    floor["kernel_us"] = round(move_ms * 1e3, 2)
+    floor["frac_of_floor"] = round(floor["random_order_us"] / (move_ms * 1e3), 3)
+    log(f"scatter floor: {floor}")
+    stages = stage_timeline(ctx, restore, scrub, lambda f: ctx.remove(A, R, flags=f, config=cfg))
+    log(f"stage timeline (us): {stages}")
     result = {
         "metric": METRIC,
         "value": round(n / (t_med * 1e-3) / 1e9, 2),
@@ -365,11 +468,14 @@ def main():
                    "path": PATH_OF_VARIANT.get(used.variant, "rz_remove"), "launch_shape": used.as_tuple()},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes": B_kernel, "avg_ms": round(move_ms, 5)},
+                     "algorithmic_bytes": B_kernel, "avg_ms": round(move_ms, 5), "library": library_sha16(),
+                     "scatter_floor": floor, **({"traffic_note": traffic_note} if traffic is None else {})},
         "op": {"algorithmic_bytes": B, "GBps": round(op_gbs, 1), "hbm_frac": round(op_gbs / HBM_PEAK_GBS, 4),
                "removals_per_s_G": round(k / (t_med * 1e-3) / 1e9, 3), "bit_exact": bit_exact,
+               "device_error_bits": device_errors,
                "kernel_us": {"mark": round(mark_ms * 1e3, 2), "move": round(move_ms * 1e3, 2),
                              "pair": round(pair_ms * 1e3, 2)},
+               "stages_us": stages,
                "wall_ms_per_step_incl_restore": round(wall * 1e3 / (args.steps + args.warmup), 3)},
     }
 
@@ -395,8 +501,18 @@ def main():
         t_on2, _, _ = time_calls(torch, ctx, restore2, lambda: ctx.partition(A, R2w), max(3, args.steps // 4), 1,
                                  scrub, False)
         rz2, on2 = statistics.median(t_rz2), statistics.median(t_on2)
+        stages2 = stage_timeline(ctx, restore2, scrub, lambda f: ctx.remove(A, R2w, flags=f, config=cfg))
+        # what bounds the ratio: one kernel launch of no work, and the bare scatter of this list's holes
+        one = torch.zeros(1, dtype=torch.int32, device="cuda")
+        t_launch, _, _ = time_calls(torch, ctx, lambda: None, lambda: one.add_(1), args.steps, 2, scrub, False)
+        floor2 = scatter_floor(torch, A, A0, R2, n, k2, restore2, scrub, args.steps, time_calls)
+        launch_us = statistics.median(t_launch) * 1e3
         extra["k0.1pct"] = {"k": k2, "redzone_ms": round(rz2, 4), "on_partition_ms": round(on2, 4),
-                            "redzone_vs_on": round(on2 / rz2, 1)}
+                            "redzone_vs_on": round(on2 / rz2, 1),
+                            "launch_shape": (cfg or ctx.table(s, k2.bit_length() - 1)).as_tuple(),
+                            "stages_us": stages2, "one_kernel_launch_us": round(launch_us, 2),
+                            "scatter_floor": floor2,
+                            "on_vs_scatter_floor": round(on2 * 1e3 / floor2["random_order_us"], 1)}
         # warm-cache figure of the headline call
         t_warm, _, _ = time_calls(torch, ctx, restore, lambda: ctx.remove(A, R, config=cfg), args.steps, 2, None, False)
         extra["warm_ms"] = round(statistics.median(t_warm), 4)
