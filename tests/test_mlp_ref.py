@@ -54,3 +54,27 @@ def test_features_by_hand():
     # empty removal list
     f = M.features(np.array([], np.int64), n, 16)
     assert f[1] == 0.0 and f[3] == -1.0 and f[5] == 0.0
+
+
+def test_features_scales_and_spread_by_hand():
+    """f[0] = log2 n / 32, f[1] = log2 k / 32, f[3] = log2(k/n) / 20 and f[7],
+    the population standard deviation of position / n, on lists worked out by
+    hand; and the strided 256-entry sample of a longer list."""
+    n = 1000
+    f = M.features(np.array([996, 997, 998, 999]), n, 8)
+    assert abs(f[0] - 9.965784284662087 / 32) < 1e-12  # log2(1000)
+    assert abs(f[1] - 2 / 32) < 1e-12
+    assert abs(f[3] - (-7.965784284662087) / 20) < 1e-12  # log2(4/1000)
+    # deviations from 0.9975: -0.0015, -0.0005, 0.0005, 0.0015 -> variance 1.25e-6
+    assert abs(f[7] - 1.25e-6 ** 0.5) < 1e-12
+    f = M.features(np.array([900, 500, 100]), n, 4)
+    assert abs(f[1] - 1.584962500721156 / 32) < 1e-12  # log2(3)
+    assert abs(f[3] - (-8.380821783940931) / 20) < 1e-12  # log2(3/1000)
+    # positions 0.9, 0.5, 0.1: mean 0.5, variance (0.16 + 0 + 0.16) / 3
+    assert abs(f[6] - 0.5) < 1e-12 and abs(f[7] - (0.32 / 3) ** 0.5) < 1e-12
+    # k = 512 > 256: the sample is every other entry, R[0], R[2], ..., R[510]
+    f = M.features(np.arange(512), 1024, 16)
+    assert abs(f[0] - 10 / 32) < 1e-12 and abs(f[1] - 9 / 32) < 1e-12 and abs(f[3] - (-1 / 20)) < 1e-12
+    assert abs(f[6] - 255 / 1024) < 1e-12
+    assert abs(f[7] - 2 * ((256 ** 2 - 1) / 12) ** 0.5 / 1024) < 1e-12  # 2 x std of 0..255
+    assert f[5] == 0.0 and f[8] == 1.0 and f[9] == 1.0 and f[10] == 1.0
