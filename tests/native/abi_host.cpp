@@ -95,10 +95,27 @@ int main() {
     HIPCHECK(hipMemcpy(dA, A.data(), n * 4, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(dR, R.data(), k * 4, hipMemcpyHostToDevice));
     CHECK(rz_remove_u32(ctx, dA, n, dR, k, 0, s) == RZ_OK, "rz_remove_u32 n=%lld", (long long)n);
-    HIPCHECK(hipStreamSynchronize(s));
+    uint32_t bits = 1;
+    CHECK(rz_check_errors(ctx, s, &bits) == RZ_OK && bits == 0, "device error bits %#x", bits);
     std::vector<uint32_t> out(n);
     HIPCHECK(hipMemcpy(out.data(), dA, n * 4, hipMemcpyDeviceToHost));
     check_redzone(out, n, R, "rz_remove_u32");
+
+    // 1b. the allocation-free entry point on the same list: no context, a
+    // caller-owned error word
+    uint32_t* dErr;
+    HIPCHECK(hipMalloc(&dErr, 4));
+    HIPCHECK(hipMemset(dErr, 0, 4));
+    HIPCHECK(hipMemcpy(dA, A.data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(dR, R.data(), k * 4, hipMemcpyHostToDevice));
+    CHECK(rz_remove_noalloc(dA, n, 4, dR, k, 0, dErr, s) == RZ_OK, "rz_remove_noalloc n=%lld", (long long)n);
+    HIPCHECK(hipStreamSynchronize(s));
+    uint32_t err = 1;
+    HIPCHECK(hipMemcpy(&err, dErr, 4, hipMemcpyDeviceToHost));
+    CHECK(err == 0, "rz_remove_noalloc error bits %#x", err);
+    HIPCHECK(hipMemcpy(out.data(), dA, n * 4, hipMemcpyDeviceToHost));
+    check_redzone(out, n, R, "rz_remove_noalloc");
+    HIPCHECK(hipFree(dErr));
 
     // 2. the O(n) partition on the same list: stable, so the survivors are the
     // kept indices in ascending order
@@ -159,6 +176,8 @@ int main() {
     HIPCHECK(hipMemcpy(dR, dup, sizeof(dup), hipMemcpyHostToDevice));
     CHECK(rz_remove_u32(ctx, dA, 3, dR, 4, 0, s) == RZ_EINVAL, "k > n must be RZ_EINVAL");
     CHECK(rz_remove_u32(ctx, dA, 64, dR, 4, RZ_VALIDATE, s) == RZ_EDUP, "a duplicate must be RZ_EDUP");
+    CHECK(rz_remove_noalloc(dA, 1 << 23, 4, dR, RZ_NOALLOC_MAX_K + 1, 0, nullptr, s) == RZ_ETOOBIG,
+          "k above the fixed scratch must be RZ_ETOOBIG");
     HIPCHECK(hipFree(dA));
     HIPCHECK(hipFree(dR));
   }
