@@ -198,9 +198,10 @@ rz_status rz_last_profile(rz_ctx* ctx, float ms[4]);
  * reference's per-phase clock64() stamps (redzone.hxx:629, 708-712, 847-851,
  * 936-939).  out[0] last mark workgroup done, out[1] ready words raised,
  * out[2] / out[3] first / last slice started, then the last slice to finish
- * each stage: out[4] ready wait, out[5] moves, out[6] look-back, out[7]
- * orphan exchange, out[8] exit; out[9..13] the mean time a slice spends in
- * each of those five stages; out[14] mark workgroups, out[15] slices.
+ * each stage: out[4] ready wait, out[5] look-back (made by the last step
+ * before that step's moves), out[6] moves, out[7] orphan exchange, out[8]
+ * exit; out[9..13] the mean time a slice spends in each of those five stages
+ * (from the end of the one before); out[14] mark workgroups, out[15] slices.
  * Synchronises the device.  RZ_EINVAL if the last call had no stamps. */
 rz_status rz_last_stages(rz_ctx* ctx, float out[16]);
 

@@ -221,8 +221,8 @@ class Context:
         return list(ms)
 
     STAGE_NAMES = ("marks_done", "ready_raised", "first_slice_start", "last_slice_start", "ready_wait_end",
-                   "moves_end", "lookback_end", "exchange_end", "exit", "mean_ready_wait", "mean_moves",
-                   "mean_lookback", "mean_exchange", "mean_exit_tail", "mark_groups", "slices")
+                   "lookback_end", "moves_end", "exchange_end", "exit", "mean_ready_wait", "mean_lookback",
+                   "mean_moves", "mean_exchange", "mean_exit_tail", "mark_groups", "slices")
 
     def last_stages(self):
         """Stage timeline (us from the first workgroup's start) of the last

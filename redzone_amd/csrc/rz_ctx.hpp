@@ -23,12 +23,14 @@ enum CtrlWord : int {
   kCtrlValidate = 192,    // validation flags: 1 = out of range, 2 = duplicate, 4 = not ascending
   kCtrlMoveEpoch = 224,   // call counter of the fused move-and-pair form (bumped by its last workgroup)
   kCtrlMoveExit = 240,    // workgroups of the fused kernel that have finished (reset by the last)
-  kCtrlMarkDone = 256,    // mark workgroups of the one-launch form that are done (reset by the last of them)
+  kCtrlMarkDone = 256,    // groups of mark workgroups of the one-launch form that are done (reset by the last)
   kCtrlMarkReady = 288,   // 8 copies, one line apart (one per XCD): move-pair epoch + 1 once every mark is in
   kCtrlGatherExit = 544,  // tiles of the one-pass gather that have finished (reset by the last)
-  kCtrlSpinCap = 576,     // test hook: polls before a bounded wait gives up (0 = the wait's own limit)
-  kCtrlStall = 577,       // test hook: nonzero makes the Redzone waits see their condition as unmet
-  kCtrlWords = 608
+  kCtrlMarkGroup = 576,   // 8 copies, one line apart: mark workgroups of the one-launch form done, per
+                          // group of ids equal mod 8 (reset by the last of the group)
+  kCtrlSpinCap = 832,     // test hook: polls before a bounded wait gives up (0 = the wait's own limit)
+  kCtrlStall = 833,       // test hook: nonzero makes the Redzone waits see their condition as unmet
+  kCtrlWords = 864
 };
 
 enum ErrorBits : uint32_t {
@@ -52,8 +54,8 @@ __device__ __forceinline__ bool stalled(const uint32_t* ctrl) {
 enum StampIndex : int {
   kStampStart = 0,     // the workgroup started
   kStampReady = 1,     // slice: the marks of this launch are in (one-launch form)
-  kStampMoves = 2,     // slice: every step's moves issued, its counts published
-  kStampLookback = 3,  // slice: the look-back has the earlier slices' orphan counts
+  kStampLookback = 2,  // slice: the look-back (in the last step) has the earlier slices' orphan counts
+  kStampMoves = 3,     // slice: every step's moves issued
   kStampExchange = 4,  // slice: every orphan of the slice went through its exchange word
   kStampExit = 5,      // slice: left (after clearing its bitmap words and counting its exit)
   kStampMarked = 1,    // mark workgroup: its marks are done

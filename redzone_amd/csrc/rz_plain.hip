@@ -165,6 +165,52 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
     }
   };
   uint32_t carry_s = 0, carry_d = 0;
+  // Exclusive orphan counts of the earlier slices, by decoupled look-back,
+  // into s_excl (wave 0).  The slice's own aggregate is published first, and
+  // a slice never waits on a later one.
+  auto lookback = [&]() {
+    const uint32_t mm = early ? min(carry_d, carry_s) : 0u;
+    const uint64_t agg = ((uint64_t)(carry_d - mm) << 31) | (uint64_t)(carry_s - mm);
+    uint64_t excl = 0;
+    if (slice > 0) {
+      int64_t j = (int64_t)slice - 1;
+      uint32_t spins = 0;
+      const uint32_t cap = spin_cap(ctrl, 1u << 24);
+      const bool stall = stalled(ctrl);
+      while (true) {
+        const int64_t q = j - lane;
+        const uint64_t w = q >= 0 ? ld_relaxed(&st[q]) : kStIncl;
+        const uint32_t flag = (uint32_t)(w >> 62);
+        const uint64_t incl_m = ballot(flag == 2u);
+        const uint64_t none_m = ballot(flag == 0u);
+        // lanes up to and including the nearest inclusive prefix must all be ready
+        const uint64_t upto = incl_m ? ((incl_m & (~incl_m + 1)) << 1) - 1 : ~0ull;
+        if (stall || (none_m & upto)) {
+          // a give-up leaves excl short: the ranks below stay inside the
+          // exchange words and every move below checks its indices
+          if (++spins > cap) {
+            if (lane == 0) or_relaxed(&ctrl[kCtrlError], kErrLookbackTimeout);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        excl += wave_sum(((upto >> lane) & 1ull) ? (w & kStCount) : 0ull);
+        if (incl_m) break;
+        j -= kWave;
+      }
+      if (lane == 0) st_relaxed(&st[slice], kStIncl | (excl + agg));
+    }
+    if (lane == 0) {
+      s_excl = excl;
+      if (slice == ns - 1) {
+        const uint64_t tot = excl + agg;
+        const uint32_t td = (uint32_t)(tot >> 31), ts = (uint32_t)tot & 0x7fffffffu;
+        if (td != ts) or_relaxed(&ctrl[kCtrlError], kErrOrphanMismatch);
+        if (stats) st_relaxed((uint64_t*)&ctrl[kCtrlStatOrphans], (uint64_t)min(td, ts));
+      }
+    }
+  };
   for (uint32_t base = a; base < b; base += WG * IPT) {
     uint32_t d[IPT], wv[IPT], info[IPT];
     T z[IPT];
@@ -235,6 +281,14 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
     const uint32_t tot = s_off[E];
     carry_s += tot & 0xffffu;
     carry_d += tot >> 16;
+    if (wave == 0 && base + WG * IPT >= b) {
+      // The last step: wave 0 looks back before it issues its own moves.  A
+      // wave's loads wait (vmcnt) for its earlier scattered stores to drain,
+      // so a look-back after the moves would start only once they had; the
+      // other waves' moves drain meanwhile.
+      lookback();
+      if (my_stamps && tid == 0) my_stamps[kStampLookback] = wall_clock64();
+    }
 #pragma unroll
     for (int j = 0; j < IPT; ++j)
       if ((info[j] & 0xffu) == kMove) A[d[j]] = z[j];
@@ -250,51 +304,7 @@ __device__ __forceinline__ void move_pair_body(T* __restrict__ A, const int32_t*
     if (dst < sZ && src < k) A[dst] = Z[src];
   }
   const uint32_t rd = carry_d - m, rs = carry_s - m;
-  // exclusive orphan counts of the earlier slices
-  if (wave == 0) {
-    const uint64_t agg = ((uint64_t)rd << 31) | rs;
-    uint64_t excl = 0;
-    if (slice > 0) {
-      int64_t j = (int64_t)slice - 1;
-      uint32_t spins = 0;
-      const uint32_t cap = spin_cap(ctrl, 1u << 24);
-      const bool stall = stalled(ctrl);
-      while (true) {
-        const int64_t q = j - lane;
-        const uint64_t w = q >= 0 ? ld_relaxed(&st[q]) : kStIncl;
-        const uint32_t flag = (uint32_t)(w >> 62);
-        const uint64_t incl_m = ballot(flag == 2u);
-        const uint64_t none_m = ballot(flag == 0u);
-        // lanes up to and including the nearest inclusive prefix must all be ready
-        const uint64_t upto = incl_m ? ((incl_m & (~incl_m + 1)) << 1) - 1 : ~0ull;
-        if (stall || (none_m & upto)) {
-          // a give-up leaves excl short: the ranks below stay inside the
-          // exchange words and every move below checks its indices
-          if (++spins > cap) {
-            if (lane == 0) or_relaxed(&ctrl[kCtrlError], kErrLookbackTimeout);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-          continue;
-        }
-        excl += wave_sum(((upto >> lane) & 1ull) ? (w & kStCount) : 0ull);
-        if (incl_m) break;
-        j -= kWave;
-      }
-      if (lane == 0) st_relaxed(&st[slice], kStIncl | (excl + agg));
-    }
-    if (lane == 0) {
-      s_excl = excl;
-      if (slice == ns - 1) {
-        const uint64_t tot = excl + agg;
-        const uint32_t td = (uint32_t)(tot >> 31), ts = (uint32_t)tot & 0x7fffffffu;
-        if (td != ts) or_relaxed(&ctrl[kCtrlError], kErrOrphanMismatch);
-        if (stats) st_relaxed((uint64_t*)&ctrl[kCtrlStatOrphans], (uint64_t)min(td, ts));
-      }
-    }
-  }
-  __syncthreads();
-  if (my_stamps && tid == 0) my_stamps[kStampLookback] = wall_clock64();
+  __syncthreads();  // the early pairs have read their LDS entries
 
   // meet the partners
   const uint64_t excl = s_excl;
@@ -380,16 +390,27 @@ __global__ __launch_bounds__(WG) void rz_mark_move_pair(T* __restrict__ A, const
       else
         mark_body<false, 16>(R + lo, hi - lo, k, sZ, bits, nullptr, threadIdx.x, WG);
     }
-    // the last mark workgroup to finish resets the count and raises the 8
-    // ready words (one per XCD, so that the waiting slices poll 8 lines)
+    // Done counts in two levels: the mark workgroups with ids equal mod 8
+    // share a group counter, the last of each group adds one to the global
+    // count, and the last group resets it and raises the 8 ready words (one
+    // per XCD, so that the waiting slices poll 8 lines).  One counter for all
+    // of them serialises: same-address atomics take about 11 ns each.
     wait_all_memory();
     __syncthreads();
     if (my_stamps && threadIdx.x == 0) my_stamps[kStampMarked] = wall_clock64();
-    if (threadIdx.x == 0 && add_relaxed(&ctrl[kCtrlMarkDone], 1u) == nm - 1u) {
-      st_relaxed(&ctrl[kCtrlMarkDone], 0u);
-      const uint32_t ready = ld_relaxed(&ctrl[kCtrlMoveEpoch]) + 1u;  // no slice can have finished yet
-      for (int x = 0; x < 8; ++x) st_relaxed(&ctrl[kCtrlMarkReady + 32 * x], ready);
-      if (my_stamps) my_stamps[kStampRaised] = wall_clock64();
+    if (threadIdx.x == 0) {
+      const uint32_t x = blockIdx.x & 7u;
+      const uint32_t gsize = (nm - x + 7u) / 8u;  // mark workgroups with id = x mod 8
+      if (add_relaxed(&ctrl[kCtrlMarkGroup + 32 * x], 1u) == gsize - 1u) {
+        st_relaxed(&ctrl[kCtrlMarkGroup + 32 * x], 0u);
+        const uint32_t groups = nm < 8u ? nm : 8u;
+        if (add_relaxed(&ctrl[kCtrlMarkDone], 1u) == groups - 1u) {
+          st_relaxed(&ctrl[kCtrlMarkDone], 0u);
+          const uint32_t ready = ld_relaxed(&ctrl[kCtrlMoveEpoch]) + 1u;  // no slice can have finished yet
+          for (int r = 0; r < 8; ++r) st_relaxed(&ctrl[kCtrlMarkReady + 32 * r], ready);
+          if (my_stamps) my_stamps[kStampRaised] = wall_clock64();
+        }
+      }
     }
     return;
   }
@@ -741,7 +762,7 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
 // the fused forms on scratch the caller provides (rz_remove_noalloc)
 // ---------------------------------------------------------------------------
 // Clears what a fused call needs at zero: the tail bitmap words of this k, the
-// error word and the two exit counters.  A kernel, not a memset, so that the
+// error word, the exit counter and the mark-done counters.  A kernel, not a memset, so that the
 // whole call can be captured in a graph.
 __global__ __launch_bounds__(256) void rz_static_prologue(uint32_t* __restrict__ bits, uint32_t nwords,
                                                           uint32_t* __restrict__ ctrl) {
@@ -752,6 +773,7 @@ __global__ __launch_bounds__(256) void rz_static_prologue(uint32_t* __restrict__
     ctrl[kCtrlMoveExit] = 0u;
     ctrl[kCtrlMarkDone] = 0u;
   }
+  if (i < 8) ctrl[kCtrlMarkGroup + 32 * i] = 0u;
 }
 
 template <typename T>

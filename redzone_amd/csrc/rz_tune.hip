@@ -92,6 +92,7 @@ static const rz_tile_config kCandidates[] = {
     {256, 4, 512, 4, 0, 1},   {256, 4, 2048, 4, 0, 1},  {256, 8, 512, 4, 0, 1},  {512, 8, 4096, 4, 0, 1},   // fused
     {256, 2, 4096, 4, 0, 1},  {256, 8, 4096, 4, 0, 1},
     {256, 4, 512, 6, 10, 1},  {256, 2, 4096, 6, 10, 1}, {256, 4, 512, 6, 12, 1}, {256, 8, 512, 6, 14, 1},  // one launch
+    {512, 4, 512, 6, 10, 1},  {512, 4, 512, 6, 12, 1},  {256, 4, 1024, 6, 11, 1},
     {1024, 8, 1024, 5, 11, 2}, {1024, 8, 1024, 5, 12, 2}, {1024, 4, 1024, 5, 11, 2}, {512, 8, 2048, 5, 11, 2},  // dense
     {1024, 8, 1024, 5, 11, 0},
 };

@@ -465,9 +465,9 @@ def test_stage_stamps(ctx, oracle, variant):
     assert st["slices"] == (k + 1023) // 1024  # one step of 256 x 4 pairs per slice at this k
     assert st["mark_groups"] == ((k + 4095) // 4096 if variant == 6 else 0)
     assert 0 <= st["first_slice_start"] <= st["last_slice_start"]
-    assert 0 < st["moves_end"] <= st["lookback_end"] <= st["exchange_end"] <= st["exit"] < 10_000
+    assert 0 < st["lookback_end"] <= st["moves_end"] <= st["exchange_end"] <= st["exit"] < 10_000
     if variant == 6:
-        assert 0 < st["marks_done"] <= st["ready_raised"] <= st["ready_wait_end"] <= st["moves_end"]
+        assert 0 < st["marks_done"] <= st["ready_raised"] <= st["ready_wait_end"] <= st["lookback_end"]
     gpu_remove_parity(ctx, oracle, A, R, config=rz.TileConfig(256, 4, 64, 0, 0, 0), flags=rz.RZ_PROFILE)
     with pytest.raises(rz.RedzoneError):
         ctx.last_stages()
