@@ -730,11 +730,14 @@ bool dense_bucket_marks(const rz_tile_config& cfg, int64_t k, uint32_t flags) {
   return cfg.variant == 5 && cfg.chunk == 2 && k <= ((int64_t)kCoarseMax << kTailBinLog) && !(flags & RZ_SORTED);
 }
 
-template <typename T>
-rz_status dense_moves(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, int64_t k, const rz_tile_config& cfg,
-                      uint32_t* bits, uint32_t* ctrl, int stats, bool bucket_marks, hipStream_t s) {
+// The dense form's bins and tiles for (cfg, n, k); RZ_EINVAL when the shape
+// cannot be launched (bin counts above their limits, LDS above the device's,
+// or a workgroup shape with no kernel).  No side effect, so a caller can
+// check before it launches anything.
+static rz_status dense_shape(const rz_ctx* ctx, const rz_tile_config& cfg, uint32_t esz, int64_t n, int64_t k,
+                             bool bucket_marks, DenseShape* out) {
+  DenseShape& d = *out;
   const int64_t sZ = n - k;
-  DenseShape d;
   slice_shape(k, cfg, kMaxSlices, &d.L, &d.nt);
   d.bs = bin_log(sZ, kCoarseTarget);
   d.NB = (uint32_t)((sZ + ((int64_t)1 << d.bs) - 1) >> d.bs);
@@ -754,10 +757,28 @@ rz_status dense_moves(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, int64_t k,
   // G = CUs per XCD keeps about two fine bins of each XCD in flight
   d.G = (uint32_t)(ctx->cu_count / 8 > 0 ? ctx->cu_count / 8 : 1);
   if (d.NB > (uint32_t)kCoarseMax || d.NBf > (uint32_t)kFineMax || d.NBt > (uint32_t)kCoarseMax) return RZ_EINVAL;
+  const size_t lds = (size_t)cfg.wg_size * cfg.items * (esz + 6) + 48 * 1024;  // + static LDS (refine)
+  if (lds > (size_t)ctx->lds_max) return RZ_EINVAL;
+  const bool kernel = (cfg.wg_size == 256 || cfg.wg_size == 512 || cfg.wg_size == 1024) &&
+                      (cfg.items == 4 || cfg.items == 8);
+  return kernel ? RZ_OK : RZ_EINVAL;
+}
+
+bool dense_shape_ok(const rz_ctx* ctx, const rz_tile_config& cfg, uint32_t esz, int64_t n, int64_t k, uint32_t flags) {
+  DenseShape d;
+  return dense_shape(ctx, cfg, esz, n, k, dense_bucket_marks(cfg, k, flags), &d) == RZ_OK;
+}
+
+template <typename T>
+rz_status dense_moves(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, int64_t k, const rz_tile_config& cfg,
+                      uint32_t* bits, uint32_t* ctrl, int stats, bool bucket_marks, hipStream_t s) {
+  const int64_t sZ = n - k;
+  DenseShape d;
+  rz_status st = dense_shape(ctx, cfg, (uint32_t)sizeof(T), n, k, bucket_marks, &d);
+  if (st != RZ_OK) return st;
   const uint32_t src_cap = (uint32_t)(k / 2 + 1);
   const uint32_t nbmax = d.NB > d.NBt ? d.NB : d.NBt;
   const size_t fcap = (size_t)d.NBf << d.bsf;
-  rz_status st;
   if ((st = ensure(ctx, ctx->dense_cnt, (size_t)d.nt * nbmax * 4 * 2, s)) != RZ_OK) return st;
   if ((st = ensure(ctx, ctx->dense_small, kSmallBytes, s)) != RZ_OK) return st;
   if ((st = ensure(ctx, ctx->dense_ent, (((size_t)k * 4 + 255) & ~(size_t)255) + (size_t)k * sizeof(T), s)) != RZ_OK)
@@ -767,8 +788,6 @@ rz_status dense_moves(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, int64_t k,
   if ((st = ensure(ctx, ctx->dense_src, (((size_t)src_cap * sizeof(T) + 255) & ~(size_t)255) + (size_t)src_cap * 4,
                    s)) != RZ_OK)
     return st;
-  const size_t lds = (size_t)cfg.wg_size * cfg.items * (sizeof(T) + 6) + 48 * 1024;  // + static LDS (refine)
-  if (lds > (size_t)ctx->lds_max) return RZ_EINVAL;
 #define RZ_DENSE_CASE(W, I) \
   if (cfg.wg_size == W && cfg.items == I) return dense_launch<T, W, I>(ctx, A, R, (uint32_t)k, (uint32_t)sZ, d, bits, ctrl, stats, src_cap, s);
   RZ_DENSE_CASE(256, 4) RZ_DENSE_CASE(256, 8) RZ_DENSE_CASE(512, 4) RZ_DENSE_CASE(512, 8)

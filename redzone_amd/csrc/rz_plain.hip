@@ -716,7 +716,11 @@ static rz_status plain_impl(rz_ctx* ctx, T* A, int64_t n, int32_t* R, int64_t k,
   RZ_HIP(hipGetLastError());
   if ((st = prof_mark(ctx, flags, 1, s)) != RZ_OK) return st;
   if (cfg.variant == 5) {
-    if ((st = dense_moves<T>(ctx, A, n, R, k, cfg, bits, ctrl, stats, dense_marks, s)) != RZ_OK) return st;
+    if ((st = dense_moves<T>(ctx, A, n, R, k, cfg, bits, ctrl, stats, dense_marks, s)) != RZ_OK) {
+      // the marks are in but no move step will clear them: leave the bitmap zero
+      if (!premarked) (void)fill_u32(bits, 0u, (size_t)((k + 31) / 32), s);
+      return st;
+    }
     if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;
     return prof_mark(ctx, flags, 3, s);
   }
@@ -824,6 +828,21 @@ rz_status launch_fused_static(void* data, int64_t n, uint32_t esz, const int32_t
   }
 }
 
+// Whether a launch shape can run, checked before any kernel of a call is
+// launched: a call that marked the tail and then refused its move step would
+// leave the bitmap set for the next call.
+static bool plain_shape_ok(const rz_ctx* ctx, const rz_tile_config& cfg, uint32_t esz, int64_t n, int64_t k,
+                           uint32_t flags) {
+  const bool wg_ok = cfg.wg_size == 256 || cfg.wg_size == 512;
+  switch (cfg.variant) {
+    case 0: case 1: case 3: case 4: case 6:
+      return wg_ok && (cfg.items == 2 || cfg.items == 4 || cfg.items == 8 || cfg.items == 16);
+    case 2: return wg_ok && (cfg.items == 4 || cfg.items == 8);
+    case 5: return dense_shape_ok(ctx, cfg, esz, n, k, flags);
+    default: return false;
+  }
+}
+
 rz_status launch_plain(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, int32_t* R, int64_t k, uint32_t flags,
                        const rz_tile_config& cfg_in, hipStream_t s, uint32_t* ext_bits) {
   rz_tile_config cfg = cfg_in;
@@ -831,6 +850,7 @@ rz_status launch_plain(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, int32_t
   // the single-launch forms include their own mark step: with the marks set
   // by the caller, run their shape as the two-kernel fused form
   if (premarked && ((cfg.variant >= 1 && cfg.variant <= 3) || cfg.variant == 6)) cfg.variant = 4;
+  if (!plain_shape_ok(ctx, cfg, esz, n, k, flags)) return RZ_EINVAL;
   if (cfg.variant == 2) return launch_queue_single(ctx, data, n, esz, R, k, flags, cfg, s);
   const bool coop = !premarked && (cfg.variant == 1 || (flags & RZ_COOPERATIVE));
   switch (esz) {

@@ -90,3 +90,25 @@ def test_orphan_mismatch_is_reported(ctx, variant):
     with pytest.raises(rz.RedzoneError) as e:
         ctx.remove(A, R, config=cfg, flags=rz.RZ_VALIDATE)
     assert e.value.status == 3
+
+
+def test_refused_shape_runs_nothing(ctx, oracle):
+    """A launch shape the library cannot run is refused before any kernel: no
+    data touched, and no tail mark left behind for the next call (the dense
+    form's 1024 x 8 staging does not fit LDS for 16-byte items; a workgroup
+    size or item count with no kernel; an unknown variant)."""
+    n, k = (1 << 20) + 3, 40_000
+    A = W.payload(16, n, seed=12)
+    R = W.uniform_removals(n, k, seed=13)
+    dA, dR = A.cuda(), R.cuda()
+    for bad in (rz.TileConfig(1024, 8, 1024, 5, 11, 0), rz.TileConfig(128, 4, 512, 4, 0, 0),
+                rz.TileConfig(256, 3, 512, 0, 0, 0), rz.TileConfig(256, 4, 512, 9, 0, 0)):
+        announce("refused shape", bad)
+        with pytest.raises(rz.RedzoneError) as e:
+            ctx.remove(dA, dR, config=bad)
+        assert e.value.status == 1
+    torch.cuda.synchronize()
+    assert torch.equal(dA.cpu(), A) and torch.equal(dR.cpu(), R)
+    assert ctx.check_errors() == 0
+    # the three-kernel form reads the tail bitmap: a mark left behind would show here
+    gpu_remove_parity(ctx, oracle, A, R, config=rz.TileConfig(256, 4, 64, 0, 0, 0))
