@@ -516,6 +516,14 @@ def main():
         # warm-cache figure of the headline call
         t_warm, _, _ = time_calls(torch, ctx, restore, lambda: ctx.remove(A, R, config=cfg), args.steps, 2, None, False)
         extra["warm_ms"] = round(statistics.median(t_warm), 4)
+        # the same call with n and k read on the device (rz_remove_sized, one-launch fused form)
+        d_n = torch.tensor([n], dtype=torch.int32, device="cuda")
+        d_k = torch.tensor([k], dtype=torch.int32, device="cuda")
+        t_sz, _, _ = time_calls(torch, ctx, restore, lambda: ctx.remove_sized(A, R, d_n, d_k), args.steps, 2, scrub,
+                                False)
+        sz_med = statistics.median(t_sz)
+        extra["device_sized"] = {"ms": round(sz_med, 4), "vs_remove": round(sz_med / t_med, 3),
+                                 "bit_exact": bool(np.array_equal(A.cpu().numpy()[: n - k], want[: n - k]))}
         result["comparators"] = extra
         log(f"comparators: {extra}")
 

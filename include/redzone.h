@@ -140,6 +140,8 @@ typedef struct rz_stats {
 #define RZ_ERR_WAIT_TIMEOUT 0x2u    /* an in-kernel wait on other workgroups (grid barrier, mark count, ready
                                        word) gave up: workgroups were not dispatched as the launch assumes */
 #define RZ_ERR_LOOKBACK_TIMEOUT 0x4u /* a decoupled look-back gave up, for the same reason */
+#define RZ_ERR_SIZES 0x8u           /* rz_remove_sized: the device-resident n and k broke 0 <= k <= n <= n_cap,
+                                       k <= k_cap; the call moved nothing */
 
 /* ---- context ------------------------------------------------------------ */
 /* The context owns all device scratch (tail bitmap, slice counts, look-back
@@ -186,6 +188,26 @@ rz_status rz_remove_rec16(rz_ctx* ctx, void* d_data, int64_t n, int32_t* d_remov
 /* rz_remove with an explicit launch shape (autotuner and tests). */
 rz_status rz_remove_config(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, int32_t* d_removals,
                            int64_t k, uint32_t flags, const rz_tile_config* cfg, void* stream);
+/* rz_remove with n and k in device memory, as the reference's launch wrapper
+ * takes them (TestGPUWide(int* A, int* sizeA, int* R, int* sizeR, ...) reads
+ * *sizeA and *sizeR inside the kernel, redzone.hxx:457-465): d_n and d_k each
+ * point at one int32, written before the call in stream order (by a kernel
+ * that decides k, for instance), so no host round trip is needed, and a
+ * captured graph of the call removes whatever k the device holds at each
+ * replay.  n_cap bounds n (d_data holds n_cap items) and k_cap bounds k (d_removals
+ * holds k_cap entries, the first k of which are read); the launch and the
+ * context scratch are sized for k_cap, and the workgroups beyond what the
+ * device k needs leave at once.  The result is rz_remove's for (n, k).  Device
+ * values outside 0 <= k <= n <= n_cap, k <= k_cap move nothing and set
+ * RZ_ERR_SIZES (read by rz_check_errors).  Runs the one-launch fused form
+ * (variant 6) with the tuned shape for k_cap when that is a fused form, else
+ * 256 x 8; RZ_EINVAL if no fused shape fits k_cap (more than about 58.7 M
+ * removals: use rz_remove).  Flags: RZ_PRESERVE_REMOVALS, RZ_SORTED,
+ * RZ_ANY_PAIRING, RZ_STATS, RZ_PROFILE; RZ_VALIDATE (which needs k on the
+ * host) and RZ_COOPERATIVE are RZ_EINVAL.  A graph capture needs an eager call
+ * with the same caps first, which grows the context scratch. */
+rz_status rz_remove_sized(rz_ctx* ctx, void* d_data, int64_t n_cap, uint32_t elem_bytes, int32_t* d_removals,
+                          int64_t k_cap, const int32_t* d_n, const int32_t* d_k, uint32_t flags, void* stream);
 /* Reads the statistics recorded by the last call made with RZ_STATS. */
 rz_status rz_last_stats(rz_ctx* ctx, rz_stats* out);
 /* Kernel times (ms) of the last call made with RZ_PROFILE: ms[0] mark, ms[1]

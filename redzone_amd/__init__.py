@@ -14,7 +14,8 @@ import ctypes
 import os
 
 from . import _native
-from ._native import (RZ_COOPERATIVE, RZ_EDEVICE, RZ_ERR_LOOKBACK_TIMEOUT, RZ_ERR_ORPHAN_MISMATCH, RZ_ERR_WAIT_TIMEOUT,
+from ._native import (RZ_COOPERATIVE, RZ_EDEVICE, RZ_ERR_LOOKBACK_TIMEOUT, RZ_ERR_ORPHAN_MISMATCH, RZ_ERR_SIZES,
+                      RZ_ERR_WAIT_TIMEOUT,
                       RZ_METHOD_DENSE, RZ_METHOD_ORDERED, RZ_METHOD_PARTITION, RZ_METHOD_REDZONE, RZ_ANY_PAIRING,
                       RZ_PRESERVE_REMOVALS, RZ_PROFILE, RZ_SORTED, RZ_STATS, RZ_VALIDATE, BatchInfo, ListDesc,
                       RedzoneError, Stats, TileConfig)
@@ -22,7 +23,7 @@ from ._native import (RZ_COOPERATIVE, RZ_EDEVICE, RZ_ERR_LOOKBACK_TIMEOUT, RZ_ER
 __all__ = ["Context", "Batch", "ListDescs", "remove", "partition", "TileConfig", "Stats", "RedzoneError", "RZ_PRESERVE_REMOVALS",
            "RZ_VALIDATE", "RZ_COOPERATIVE", "RZ_STATS", "RZ_PROFILE", "RZ_SORTED", "RZ_ANY_PAIRING", "RZ_METHOD_REDZONE",
            "RZ_METHOD_ORDERED", "RZ_METHOD_DENSE", "RZ_METHOD_PARTITION", "RZ_EDEVICE", "RZ_ERR_ORPHAN_MISMATCH",
-           "RZ_ERR_WAIT_TIMEOUT", "RZ_ERR_LOOKBACK_TIMEOUT", "elem_bytes_of", "default_context"]
+           "RZ_ERR_WAIT_TIMEOUT", "RZ_ERR_LOOKBACK_TIMEOUT", "RZ_ERR_SIZES", "elem_bytes_of", "default_context"]
 
 
 def elem_bytes_of(data):
@@ -129,6 +130,24 @@ class Context:
             self._call("rz_remove_config", self._h, ctypes.c_void_p(data.data_ptr()), n, esz,
                        ctypes.c_void_p(removals.data_ptr()), k, flags, ctypes.byref(config), s)
         return n - k
+
+    def remove_sized(self, data, removals, d_n, d_k, n_cap=None, k_cap=None, flags=0, stream=None):
+        """remove() with n and k read on the device from the one-element int32
+        tensors d_n and d_k (TestGPUWide's sizeA / sizeR, redzone.hxx:457-465).
+        n_cap (default data.shape[0]) and k_cap (default removals.shape[0])
+        bound them; the first k entries of removals are read.  Returns nothing:
+        the new length n - k is on the device.  Sizes that break their bounds
+        move nothing and set RZ_ERR_SIZES (see check_errors)."""
+        import torch
+
+        n_cap, k_cap, esz = self._args(data, removals, n_cap, k_cap)
+        for t, name in ((d_n, "d_n"), (d_k, "d_k")):
+            _check_tensor(t, name, self.device)
+            if t.dtype != torch.int32 or t.numel() < 1:
+                raise ValueError(f"{name} must be an int32 tensor of at least one element")
+        self._call("rz_remove_sized", self._h, ctypes.c_void_p(data.data_ptr()), n_cap, esz,
+                   ctypes.c_void_p(removals.data_ptr()), k_cap, ctypes.c_void_p(d_n.data_ptr()),
+                   ctypes.c_void_p(d_k.data_ptr()), flags, _stream_handle(stream, self.device))
 
     def add_remove(self, data, n, additions, removals, k=None, flags=0, stream=None):
         """Appends `additions` after data[:n] (data must hold n + g items), then

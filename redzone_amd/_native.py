@@ -16,6 +16,7 @@ RZ_NOALLOC_MAX_K = 1 << 22
 RZ_ERR_ORPHAN_MISMATCH = 0x1
 RZ_ERR_WAIT_TIMEOUT = 0x2
 RZ_ERR_LOOKBACK_TIMEOUT = 0x4
+RZ_ERR_SIZES = 0x8
 # flags
 RZ_PRESERVE_REMOVALS = 0x1
 RZ_VALIDATE = 0x2
@@ -142,6 +143,7 @@ SIGNATURES = [
     ("rz_check_errors", _i32, [_vp, _vp, ctypes.POINTER(_u32)]),
     ("rz_debug_set_waits", _i32, [_vp, _u32, _u32]),
     ("rz_remove_noalloc", _i32, [_vp, _i64, _u32, _vp, _i64, _u32, _vp, _vp]),
+    ("rz_remove_sized", _i32, [_vp, _vp, _i64, _u32, _vp, _i64, _vp, _vp, _u32, _vp]),
     ("rz_last_stages", _i32, [_vp, ctypes.POINTER(ctypes.c_float)]),
 ]
 

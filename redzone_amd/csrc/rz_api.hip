@@ -245,6 +245,24 @@ rz_status rz_remove(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, i
   return rz_remove_config(ctx, d_data, n, elem_bytes, d_removals, k, flags, &cfg, stream);
 }
 
+rz_status rz_remove_sized(rz_ctx* ctx, void* d_data, int64_t n_cap, uint32_t elem_bytes, int32_t* d_removals,
+                          int64_t k_cap, const int32_t* d_n, const int32_t* d_k, uint32_t flags, void* stream) {
+  rz_status st = check_args(ctx, d_data, n_cap, elem_bytes, d_removals, k_cap);
+  if (st != RZ_OK) return st;
+  if (!d_n || !d_k || ((uintptr_t)d_n % 4) != 0 || ((uintptr_t)d_k % 4) != 0) return RZ_EINVAL;
+  const uint32_t allowed = RZ_PRESERVE_REMOVALS | RZ_SORTED | RZ_ANY_PAIRING | RZ_STATS | RZ_PROFILE;
+  if (flags & ~allowed) return RZ_EINVAL;  // RZ_VALIDATE needs k on the host; RZ_COOPERATIVE has no sized form
+  if (k_cap == 0) return RZ_OK;  // the device k can only be 0 (or break its bound, which nothing then reports)
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  DeviceGuard g(ctx->device);
+  rz_tile_config cfg;
+  if (!sized_shape(ctx, elem_bytes, k_cap, &cfg)) return RZ_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  st = launch_sized(ctx, d_data, n_cap, elem_bytes, d_removals, k_cap, d_n, d_k, flags, cfg, s);
+  if (st != RZ_OK) return st;
+  return finish_call(ctx, flags, s);
+}
+
 rz_status rz_remove_premarked(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, int32_t* d_removals,
                               int64_t k, uint32_t* d_tail_bits, uint32_t flags, const rz_tile_config* cfg,
                               void* stream) {

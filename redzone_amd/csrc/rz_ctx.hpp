@@ -37,6 +37,7 @@ enum ErrorBits : uint32_t {
   kErrOrphanMismatch = RZ_ERR_ORPHAN_MISMATCH,   // dest and source orphan totals differ (duplicate or bad input)
   kErrBarrierTimeout = RZ_ERR_WAIT_TIMEOUT,      // an in-kernel wait gave up (workgroups not co-resident)
   kErrLookbackTimeout = RZ_ERR_LOOKBACK_TIMEOUT, // a look-back wait gave up
+  kErrSizes = RZ_ERR_SIZES,                      // device-resident n, k outside the call's bounds
 };
 
 // Polls a bounded wait makes before it gives up: the control block's cap when
@@ -133,6 +134,12 @@ rz_tile_config default_config(uint32_t esz, int64_t n, int64_t k);
 rz_status launch_plain(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, int32_t* R, int64_t k, uint32_t flags,
                        const rz_tile_config& cfg, hipStream_t s, uint32_t* ext_bits = nullptr);
 rz_status launch_mark_removals(rz_ctx* ctx, const int32_t* R, int64_t n, int64_t k, uint32_t* bits, hipStream_t s);
+// rz_remove_sized: the one-launch fused shape for up to k_cap removals (false
+// when none fits), and its launch with n and k read from d_n, d_k on the device
+bool sized_shape(const rz_ctx* ctx, uint32_t esz, int64_t k_cap, rz_tile_config* out);
+rz_status launch_sized(rz_ctx* ctx, void* data, int64_t n_cap, uint32_t esz, const int32_t* R, int64_t k_cap,
+                       const int32_t* d_n, const int32_t* d_k, uint32_t flags, const rz_tile_config& cfg,
+                       hipStream_t s);
 rz_status launch_partition(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, const int32_t* R, int64_t k,
                            uint32_t flags, hipStream_t s);
 rz_status launch_validate(rz_ctx* ctx, const int32_t* R, int64_t n, int64_t k, uint32_t flags, hipStream_t s);

@@ -369,6 +369,47 @@ __global__ __launch_bounds__(WG) void rz_move_pair(T* __restrict__ A, const int3
 constexpr int kFusedMarkTileLog = 12;  // default tile: 4096 removals
 constexpr uint32_t kMarkWaitBackoff = 16;  // longest pause between polls of the ready word: 16 x 64 cycles
 
+// One mark workgroup of the one-launch form (id < nm): mark its tile of the
+// removals, then count itself done.
+template <int WG>
+__device__ __forceinline__ void mark_group_body(const int32_t* __restrict__ R, uint32_t k, uint32_t sZ,
+                                                uint32_t* __restrict__ bits, uint32_t* __restrict__ ctrl, int stats,
+                                                uint32_t nm, uint32_t tile, int sorted,
+                                                uint64_t* __restrict__ stamps) {
+  uint64_t* const my_stamps = stamps ? stamps + (size_t)blockIdx.x * kStampsPerGroup : nullptr;
+  if (my_stamps && threadIdx.x == 0) my_stamps[kStampStart] = wall_clock64();
+  const uint32_t lo = blockIdx.x * tile, hi = min(lo + tile, k);
+  if (!sorted || (uint32_t)R[hi - 1] >= sZ) {  // a sorted chunk below the red zone has no marks
+    // a whole tile's loads in flight at once (16 per thread for 4,096 removals and 256 threads)
+    if (stats)
+      mark_body<true, 16>(R + lo, hi - lo, k, sZ, bits, (uint64_t*)&ctrl[kCtrlStatTail], threadIdx.x, WG);
+    else
+      mark_body<false, 16>(R + lo, hi - lo, k, sZ, bits, nullptr, threadIdx.x, WG);
+  }
+  // Done counts in two levels: the mark workgroups with ids equal mod 8
+  // share a group counter, the last of each group adds one to the global
+  // count, and the last group resets it and raises the 8 ready words (one
+  // per XCD, so that the waiting slices poll 8 lines).  One counter for all
+  // of them serialises: same-address atomics take about 11 ns each.
+  wait_all_memory();
+  __syncthreads();
+  if (my_stamps && threadIdx.x == 0) my_stamps[kStampMarked] = wall_clock64();
+  if (threadIdx.x == 0) {
+    const uint32_t x = blockIdx.x & 7u;
+    const uint32_t gsize = (nm - x + 7u) / 8u;  // mark workgroups with id = x mod 8
+    if (add_relaxed(&ctrl[kCtrlMarkGroup + 32 * x], 1u) == gsize - 1u) {
+      st_relaxed(&ctrl[kCtrlMarkGroup + 32 * x], 0u);
+      const uint32_t groups = nm < 8u ? nm : 8u;
+      if (add_relaxed(&ctrl[kCtrlMarkDone], 1u) == groups - 1u) {
+        st_relaxed(&ctrl[kCtrlMarkDone], 0u);
+        const uint32_t ready = ld_relaxed(&ctrl[kCtrlMoveEpoch]) + 1u;  // no slice can have finished yet
+        for (int r = 0; r < 8; ++r) st_relaxed(&ctrl[kCtrlMarkReady + 32 * r], ready);
+        if (my_stamps) my_stamps[kStampRaised] = wall_clock64();
+      }
+    }
+  }
+}
+
 template <typename T, int WG, int IPT>
 __global__ __launch_bounds__(WG) void rz_mark_move_pair(T* __restrict__ A, const int32_t* __restrict__ R, uint32_t k,
                                                          uint32_t sZ, uint32_t L, uint32_t ns,
@@ -380,42 +421,53 @@ __global__ __launch_bounds__(WG) void rz_mark_move_pair(T* __restrict__ A, const
                                                          uint32_t* __restrict__ err_out,
                                                          uint64_t* __restrict__ stamps) {
   if (blockIdx.x < nm) {
-    uint64_t* const my_stamps = stamps ? stamps + (size_t)blockIdx.x * kStampsPerGroup : nullptr;
-    if (my_stamps && threadIdx.x == 0) my_stamps[kStampStart] = wall_clock64();
-    const uint32_t lo = blockIdx.x * tile, hi = min(lo + tile, k);
-    if (!sorted || (uint32_t)R[hi - 1] >= sZ) {  // a sorted chunk below the red zone has no marks
-      // a whole tile's loads in flight at once (16 per thread for 4,096 removals and 256 threads)
-      if (stats)
-        mark_body<true, 16>(R + lo, hi - lo, k, sZ, bits, (uint64_t*)&ctrl[kCtrlStatTail], threadIdx.x, WG);
-      else
-        mark_body<false, 16>(R + lo, hi - lo, k, sZ, bits, nullptr, threadIdx.x, WG);
-    }
-    // Done counts in two levels: the mark workgroups with ids equal mod 8
-    // share a group counter, the last of each group adds one to the global
-    // count, and the last group resets it and raises the 8 ready words (one
-    // per XCD, so that the waiting slices poll 8 lines).  One counter for all
-    // of them serialises: same-address atomics take about 11 ns each.
-    wait_all_memory();
-    __syncthreads();
-    if (my_stamps && threadIdx.x == 0) my_stamps[kStampMarked] = wall_clock64();
-    if (threadIdx.x == 0) {
-      const uint32_t x = blockIdx.x & 7u;
-      const uint32_t gsize = (nm - x + 7u) / 8u;  // mark workgroups with id = x mod 8
-      if (add_relaxed(&ctrl[kCtrlMarkGroup + 32 * x], 1u) == gsize - 1u) {
-        st_relaxed(&ctrl[kCtrlMarkGroup + 32 * x], 0u);
-        const uint32_t groups = nm < 8u ? nm : 8u;
-        if (add_relaxed(&ctrl[kCtrlMarkDone], 1u) == groups - 1u) {
-          st_relaxed(&ctrl[kCtrlMarkDone], 0u);
-          const uint32_t ready = ld_relaxed(&ctrl[kCtrlMoveEpoch]) + 1u;  // no slice can have finished yet
-          for (int r = 0; r < 8; ++r) st_relaxed(&ctrl[kCtrlMarkReady + 32 * r], ready);
-          if (my_stamps) my_stamps[kStampRaised] = wall_clock64();
-        }
-      }
-    }
+    mark_group_body<WG>(R, k, sZ, bits, ctrl, stats, nm, tile, sorted, stamps);
     return;
   }
   move_pair_body<T, WG, IPT, true>(A, R, k, sZ, L, ns, bits, mstatus, slots, slot_cap, ctrl, stats, early,
                                    blockIdx.x - nm, backoff, err_out, stamps);
+}
+
+// The one-launch form with n and k read from device memory (rz_remove_sized;
+// the reference's TestGPUWide dereferences sizeA and sizeR in the kernel,
+// redzone.hxx:457-465).  The grid is sized on the host for k_cap removals:
+// nm_cap mark workgroups, then ns_cap = min(cap, ceil(k_cap / step)) slices.
+// Each workgroup derives this call's shape from the device k with the host's
+// slice rule (L = whole steps so that at most `cap` slices cover k), so
+// L(k) <= L(k_cap) fits the launch's LDS and ns(k) <= ns_cap; the workgroups
+// beyond nm(k) marks and ns(k) slices leave at once and take part in no count.
+// Sizes outside 0 <= k <= n <= n_cap, k <= k_cap move nothing and set
+// kErrSizes.
+template <typename T, int WG, int IPT>
+__global__ __launch_bounds__(WG) void rz_mark_move_pair_sized(T* __restrict__ A, const int32_t* __restrict__ R,
+                                                               const int32_t* __restrict__ d_n,
+                                                               const int32_t* __restrict__ d_k, uint32_t n_cap,
+                                                               uint32_t k_cap, uint32_t cap, uint32_t nm_cap,
+                                                               uint32_t* __restrict__ bits,
+                                                               uint64_t* __restrict__ mstatus,
+                                                               uint64_t* __restrict__ slots,
+                                                               uint32_t* __restrict__ ctrl, int stats, uint32_t tile,
+                                                               uint32_t backoff, int sorted, int early) {
+  const int64_t n = d_n[0], k = d_k[0];
+  if (k < 0 || k > n || n > (int64_t)n_cap || k > (int64_t)k_cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) or_relaxed(&ctrl[kCtrlError], kErrSizes);
+    return;
+  }
+  if (k == 0 || k == n) return;  // nothing moves
+  const uint32_t uk = (uint32_t)k, sZ = (uint32_t)(n - k);
+  constexpr uint64_t step = (uint64_t)WG * IPT;
+  const uint64_t span = step * cap;
+  const uint32_t L = (uint32_t)(((uk + span - 1) / span) * step);
+  const uint32_t ns = (uk + L - 1) / L;
+  const uint32_t nm = (uint32_t)(((uint64_t)uk + tile - 1) / tile);
+  if (blockIdx.x < nm_cap) {
+    if (blockIdx.x < nm) mark_group_body<WG>(R, uk, sZ, bits, ctrl, stats, nm, tile, sorted, nullptr);
+    return;
+  }
+  const uint32_t slice = blockIdx.x - nm_cap;
+  if (slice >= ns) return;
+  move_pair_body<T, WG, IPT, true>(A, R, uk, sZ, L, ns, bits, mstatus, slots, uk / 2 + 1, ctrl, stats, early, slice,
+                                   backoff, nullptr, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -857,6 +909,90 @@ rz_status launch_plain(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, int32_t
     case 4: return plain_impl<uint32_t>(ctx, (uint32_t*)data, n, R, k, flags, cfg, coop, s, ext_bits);
     case 8: return plain_impl<uint64_t>(ctx, (uint64_t*)data, n, R, k, flags, cfg, coop, s, ext_bits);
     case 16: return plain_impl<uint4>(ctx, (uint4*)data, n, R, k, flags, cfg, coop, s, ext_bits);
+    default: return RZ_EINVAL;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// sizes in device memory (rz_remove_sized)
+// ---------------------------------------------------------------------------
+// The one-launch shape for up to k_cap removals: the context's tuned shape for
+// k_cap when it is a fused form whose slices fit LDS at k_cap, else 256 x 8 with
+// up to kMaxSlices slices.  false when no fused shape fits k_cap.
+bool sized_shape(const rz_ctx* ctx, uint32_t esz, int64_t k_cap, rz_tile_config* out) {
+  rz_tile_config c = ctx->table[esz_index(esz)][log2_floor(k_cap)];
+  uint32_t L, ns;
+  const bool tuned = (c.variant == 4 || c.variant == 6) && (c.wg_size == 256 || c.wg_size == 512) &&
+                     (c.items == 2 || c.items == 4 || c.items == 8 || c.items == 16);
+  if (tuned) {
+    if (c.variant == 4) c.slice_steps = 0;  // the default mark tile
+    c.variant = 6;
+    slice_shape(k_cap, c, kMaxSlices, &L, &ns);
+    if (fused_fits(ctx, c, L)) {
+      *out = c;
+      return true;
+    }
+  }
+  c = rz_tile_config{256, 8, kMaxSlices, 6, 0, 0};
+  slice_shape(k_cap, c, kMaxSlices, &L, &ns);
+  if (!fused_fits(ctx, c, L)) return false;
+  *out = c;
+  return true;
+}
+
+template <typename T>
+static rz_status sized_impl(rz_ctx* ctx, T* A, int64_t n_cap, const int32_t* R, int64_t k_cap, const int32_t* d_n,
+                            const int32_t* d_k, uint32_t flags, const rz_tile_config& cfg, hipStream_t s) {
+  uint32_t L_cap, ns_unused;
+  slice_shape(k_cap, cfg, kMaxSlices, &L_cap, &ns_unused);
+  const int64_t step = (int64_t)cfg.wg_size * cfg.items;
+  const uint32_t cap = (uint32_t)(cfg.max_slices < 1 ? 1 : (cfg.max_slices > kMaxSlices ? kMaxSlices : cfg.max_slices));
+  const int64_t ns_k = (k_cap + step - 1) / step;
+  const uint32_t ns_cap = (uint32_t)(ns_k < (int64_t)cap ? ns_k : (int64_t)cap);
+  const uint32_t tile = 1u << clamp_tile_log(cfg.slice_steps);
+  const uint32_t nm_cap = (uint32_t)((k_cap + tile - 1) / tile);
+  rz_status st;
+  ctx->stamp_groups = 0;
+  if ((st = ensure(ctx, ctx->tail_bits, (size_t)((k_cap + 63) / 64) * 8, s)) != RZ_OK) return st;
+  if ((st = ensure(ctx, ctx->mstatus, 2 * (size_t)kMaxSlices * 8, s)) != RZ_OK) return st;
+  if ((st = ensure(ctx, ctx->slots, ((size_t)k_cap / 2 + 1) * 8, s)) != RZ_OK) return st;
+  uint32_t* ctrl = (uint32_t*)ctx->ctrl.ptr;
+  const int stats = (flags & RZ_STATS) ? 1 : 0;
+  if (stats) {
+    if ((st = fill_u32(ctrl + kCtrlStatTail, 0u, 2, s)) != RZ_OK) return st;
+    if ((st = fill_u32(ctrl + kCtrlStatOrphans, 0u, 2, s)) != RZ_OK) return st;
+  }
+  if ((st = prof_mark(ctx, flags, 0, s)) != RZ_OK) return st;
+  if ((st = prof_mark(ctx, flags, 1, s)) != RZ_OK) return st;
+  uint32_t* bits = (uint32_t*)ctx->tail_bits.ptr;
+  uint64_t* mstatus = (uint64_t*)ctx->mstatus.ptr;
+  uint64_t* slots = (uint64_t*)ctx->slots.ptr;
+  const uint32_t bo = kMarkWaitBackoff, nc = (uint32_t)n_cap, kc = (uint32_t)k_cap;
+  const int sorted = (flags & RZ_SORTED) ? 1 : 0, early = (flags & RZ_ANY_PAIRING) ? 1 : 0;
+  const size_t lds = (size_t)L_cap * 4;
+  bool launched = false;
+#define RZ_SZ_CASE(W, I)                                                                                         \
+  if (!launched && cfg.wg_size == W && cfg.items == I) {                                                         \
+    hipLaunchKernelGGL((rz_mark_move_pair_sized<T, W, I>), dim3(nm_cap + ns_cap), dim3(W), lds, s, A, R, d_n, d_k, \
+                       nc, kc, cap, nm_cap, bits, mstatus, slots, ctrl, stats, tile, bo, sorted, early);          \
+    launched = true;                                                                                             \
+  }
+  RZ_SZ_CASE(256, 2) RZ_SZ_CASE(256, 4) RZ_SZ_CASE(256, 8) RZ_SZ_CASE(256, 16)
+  RZ_SZ_CASE(512, 2) RZ_SZ_CASE(512, 4) RZ_SZ_CASE(512, 8) RZ_SZ_CASE(512, 16)
+#undef RZ_SZ_CASE
+  if (!launched) return RZ_EINVAL;
+  RZ_HIP(hipGetLastError());
+  if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;
+  return prof_mark(ctx, flags, 3, s);
+}
+
+rz_status launch_sized(rz_ctx* ctx, void* data, int64_t n_cap, uint32_t esz, const int32_t* R, int64_t k_cap,
+                       const int32_t* d_n, const int32_t* d_k, uint32_t flags, const rz_tile_config& cfg,
+                       hipStream_t s) {
+  switch (esz) {
+    case 4: return sized_impl<uint32_t>(ctx, (uint32_t*)data, n_cap, R, k_cap, d_n, d_k, flags, cfg, s);
+    case 8: return sized_impl<uint64_t>(ctx, (uint64_t*)data, n_cap, R, k_cap, d_n, d_k, flags, cfg, s);
+    case 16: return sized_impl<uint4>(ctx, (uint4*)data, n_cap, R, k_cap, d_n, d_k, flags, cfg, s);
     default: return RZ_EINVAL;
   }
 }

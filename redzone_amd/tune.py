@@ -2,7 +2,8 @@
 """Autotunes the launch shapes on this device and writes the table that
 Context loads by default (redzone_amd/tuned_gfx950.txt).
 
-    python -m redzone_amd.tune [--out PATH] [--reps 5] [--quick]
+    python -m redzone_amd.tune [--out PATH] [--reps 5]
+    python -m redzone_amd.tune --check-list-bytes 16777216 67108864
 
 For every element size (4, 8, 16 bytes) and every k bucket [2^b, 2^(b+1))
 (up to k = n/2) the native tuner (rz_tune) times the built-in candidate
@@ -10,6 +11,12 @@ shapes at k = 2^(b+1/2) on a synthetic list of 1 GiB with cold caches and
 keeps the fastest; then the batch tuner
 (rz_tune_batch) picks the queue shape on a 4096-list batch shaped like the
 batched benchmark.  Every line of the result is printed as JSON.
+
+The table is keyed by element size and log2 k only.  --check-list-bytes
+measures what that costs on lists of other sizes: for each given list size
+(bytes) and each bucket it times every candidate on a cold list of that size
+and prints the installed shape's time next to the fastest candidate's, then
+the geometric mean of their ratio.  It writes no table.
 """
 import argparse
 import json
@@ -21,6 +28,34 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 
+def check_sizes(rz, sizes, reps):
+    import math
+
+    installed = rz.Context(0)
+    scratch = rz.Context(0, load_tuned=False)  # rz_tune installs its winner here, not in `installed`
+    for list_bytes in sizes:
+        ratios = []
+        for esz in (4, 8, 16):
+            n = list_bytes // esz
+            for b in range(8, 32):
+                k = int(2 ** (b + 0.5))
+                if k * 2 > n:
+                    break
+                entry = installed.table(esz, b).as_tuple()
+                best, ms, results = scratch.tune(esz, n, k, reps=reps, cold=True)
+                t = [tm for c, tm in results if c.as_tuple() == entry and tm > 0]
+                row = {"list_bytes": list_bytes, "elem_bytes": esz, "n": n, "log2k": b, "installed": entry,
+                       "installed_ms": round(t[0], 5) if t else None, "best": best.as_tuple(), "best_ms": round(ms, 5)}
+                if t:
+                    ratios.append(t[0] / ms)
+                    row["ratio"] = round(t[0] / ms, 3)
+                print(json.dumps(row), flush=True)
+        if ratios:
+            print(json.dumps({"list_bytes": list_bytes, "buckets": len(ratios),
+                              "geomean_ratio": round(math.exp(sum(map(math.log, ratios)) / len(ratios)), 4),
+                              "max_ratio": round(max(ratios), 3)}), flush=True)
+
+
 def main():
     import redzone_amd as rz
     from redzone_amd import _native
@@ -28,7 +63,11 @@ def main():
     p = argparse.ArgumentParser()
     p.add_argument("--out", default=_native.TUNED_TABLE)
     p.add_argument("--reps", type=int, default=5)
+    p.add_argument("--check-list-bytes", type=int, nargs="*", default=None)
     args = p.parse_args()
+    if args.check_list_bytes:
+        check_sizes(rz, args.check_list_bytes, args.reps)
+        return
     ctx = rz.Context(0, load_tuned=False)
     print(json.dumps({"device": ctx.device_info()}), flush=True)
     lo_b = 8

@@ -117,6 +117,22 @@ int main() {
     check_redzone(out, n, R, "rz_remove_noalloc");
     HIPCHECK(hipFree(dErr));
 
+    // 1c. sizes in device memory (the reference's TestGPUWide(A, sizeA, R, sizeR)
+    // shape): n and k written to the device, the call sized for them
+    int32_t* dNK;
+    HIPCHECK(hipMalloc(&dNK, 8));
+    const int32_t sizes[2] = {(int32_t)n, (int32_t)k};
+    HIPCHECK(hipMemcpy(dNK, sizes, 8, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(dA, A.data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(dR, R.data(), k * 4, hipMemcpyHostToDevice));
+    CHECK(rz_remove_sized(ctx, dA, n, 4, dR, k, dNK, dNK + 1, 0, s) == RZ_OK, "rz_remove_sized n=%lld",
+          (long long)n);
+    bits = 1;
+    CHECK(rz_check_errors(ctx, s, &bits) == RZ_OK && bits == 0, "rz_remove_sized error bits %#x", bits);
+    HIPCHECK(hipMemcpy(out.data(), dA, n * 4, hipMemcpyDeviceToHost));
+    check_redzone(out, n, R, "rz_remove_sized");
+    HIPCHECK(hipFree(dNK));
+
     // 2. the O(n) partition on the same list: stable, so the survivors are the
     // kept indices in ascending order
     HIPCHECK(hipMemcpy(dA, A.data(), n * 4, hipMemcpyHostToDevice));
@@ -178,6 +194,9 @@ int main() {
     CHECK(rz_remove_u32(ctx, dA, 64, dR, 4, RZ_VALIDATE, s) == RZ_EDUP, "a duplicate must be RZ_EDUP");
     CHECK(rz_remove_noalloc(dA, 1 << 23, 4, dR, RZ_NOALLOC_MAX_K + 1, 0, nullptr, s) == RZ_ETOOBIG,
           "k above the fixed scratch must be RZ_ETOOBIG");
+    CHECK(rz_remove_sized(ctx, dA, 64, 4, dR, 4, nullptr, dR, 0, s) == RZ_EINVAL, "rz_remove_sized needs d_n");
+    CHECK(rz_remove_sized(ctx, dA, 64, 4, dR, 4, dR, dR, RZ_VALIDATE, s) == RZ_EINVAL,
+          "rz_remove_sized cannot validate (k is on the device)");
     HIPCHECK(hipFree(dA));
     HIPCHECK(hipFree(dR));
   }
