@@ -75,6 +75,9 @@ typedef struct rz_tile_config {
   int32_t max_slices;  /* variants 0/1/3/4/6: upper bound on slices (workgroups of the move step) */
   int32_t variant;     /* 0 = three kernels on the stream (mark, move, pair), 1 = one cooperative kernel,
                           2 = persistent queue, 3 = one ordered launch (roles by workgroup id),
+                          (batches take 2 = MARK, MOVE and PAIR items per list, or 4 = MARK and fused
+                          items: a list of several slices of at most 8,192 pairs each pairs its orphans
+                          inside the move items, through look-back and exchange words),
                           4 = two kernels (mark, then move with in-kernel orphan pairing; slices longer than
                               the workgroup LDS allows run as variant 0),
                           5 = dense form: head writes grouped by destination bin, then one scatter per bin

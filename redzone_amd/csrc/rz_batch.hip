@@ -8,7 +8,14 @@
 //                 redzone.hxx:236-444, on a 64-lane wavefront workgroup);
 //   MARK  j       marks of removals [jL, (j+1)L) into the list's tail bitmap;
 //   MOVE  j       slice j, after every MARK of the list has finished;
-//   PAIR  j       share j of the orphan pairs, after every MOVE has finished.
+//   PAIR  j       share j of the orphan pairs, after every MOVE has finished;
+//   FMOVE j       (fused items, batch variant 4, in place of MOVE and PAIR)
+//                 slice j after every MARK of the list has finished: moves its
+//                 pairs, keeps its orphans in LDS and pairs them through the
+//                 list's look-back words and exchange words (move_pair_core,
+//                 rz_fused_dev.hpp).  The list's MARK j also zeroes the
+//                 slice's look-back word and its share of the list's exchange
+//                 words, so no list state survives from an earlier launch.
 // A batch is planned into one queue per XCD: lists go to XCDs by longest-
 // processing-time order on their removal counts, and all items of a list sit
 // in its XCD's queue.  Workgroup w drains queue w mod 8 first (the XCD the
@@ -31,12 +38,15 @@
 #include <algorithm>
 #include <vector>
 
+#include "rz_fused_dev.hpp"
 #include "rz_redzone_dev.hpp"
 
 namespace rz {
 
 constexpr int kQueueMaxSlices = 1024;  // slices per list in the queue kernel (LDS prefix)
-enum ItemKind : uint32_t { kSingle = 0, kMark = 1, kMoveItem = 2, kPair = 3 };
+enum ItemKind : uint32_t { kSingle = 0, kMark = 1, kMoveItem = 2, kPair = 3, kFusedMove = 4 };
+// longest slice of a list with fused items (its orphans take 4 bytes per pair of LDS)
+constexpr uint32_t kFusedItemMaxL = 8192;
 
 struct ListPlan {
   uint64_t data;      // T*
@@ -46,8 +56,8 @@ struct ListPlan {
   uint32_t m, esz;
   uint32_t bits_off;    // first bitmap word of this list
   uint32_t counts_off;  // first slice-count word of this list
-  uint32_t npair;       // number of PAIR items
-  uint32_t pad;
+  uint32_t npair;       // number of PAIR items (0: the list runs fused items)
+  uint32_t slots_off;   // fused items: first exchange word of this list (k/2 + 1 words)
 };
 static_assert(sizeof(ListPlan) == 64, "ListPlan is one 64-byte record");
 
@@ -60,11 +70,14 @@ struct QueueSplit {
   uint32_t off[kQueues + 1];
 };
 
-template <typename T, int WG, int IPT>
+// FUSED: the kernel instantiation that runs fused items (batch variant 4); the
+// other one leaves their code out, which keeps its registers and occupancy.
+template <typename T, int WG, int IPT, bool FUSED>
 __device__ __forceinline__ void run_item(const ListPlan& P, uint32_t li, uint32_t kind, uint32_t j,
                                          uint32_t* __restrict__ done, uint32_t* __restrict__ bits,
-                                         uint64_t* __restrict__ counts, uint32_t* __restrict__ ctrl,
-                                         MoveShared<WG, IPT>& msh, PairShared<WG, kQueueMaxSlices>& psh) {
+                                         uint64_t* __restrict__ counts, uint64_t* __restrict__ slots,
+                                         uint32_t* __restrict__ ctrl, MoveShared<WG, IPT>& msh,
+                                         PairShared<WG, kQueueMaxSlices>& psh) {
   T* A = (T*)P.data;
   const int32_t* R = (const int32_t*)P.R;
   int32_t* Rd = (int32_t*)P.Rd;
@@ -82,7 +95,22 @@ __device__ __forceinline__ void run_item(const ListPlan& P, uint32_t li, uint32_
   } else if (kind == kMark) {
     const uint32_t a = j * P.L, b = min(a + P.L, P.k);
     mark_body<false>(R + a, b - a, P.k, P.sZ, lb, nullptr, tid, WG);
+    if (FUSED && P.npair == 0) {
+      // fused items: this slice's look-back word and a 1/m share of the
+      // list's exchange words start at zero
+      if (tid == 0) st_relaxed(&lc[j], 0ull);
+      const uint32_t nslot = P.k / 2 + 1, share = (nslot + P.m - 1) / P.m;
+      uint64_t* ls = slots + P.slots_off;
+      const uint32_t lo = j * share, hi = min(lo + share, nslot);
+      for (uint32_t w = lo + tid; w < hi; w += WG) st_relaxed(&ls[w], 0ull);
+    }
     signal_done(&done[2 * li]);
+  } else if (kind == kFusedMove) {
+    if constexpr (FUSED) {
+      wait_count(&done[2 * li], P.m, ctrl);
+      move_pair_core<T, WG, IPT, true, false>(A, R, P.k, P.sZ, P.L, P.m, lb, lc, slots + P.slots_off,
+                                              P.k / 2 + 1, ctrl, 0, 0, j, 0u, 1u, nullptr);
+    }
   } else if (kind == kMoveItem) {
     wait_count(&done[2 * li], P.m, ctrl);
     move_slice<T, WG, IPT, true>(A, R, Rd, P.k, P.sZ, P.L, j, lb, lc, msh, &done[2 * li + 1]);
@@ -94,12 +122,13 @@ __device__ __forceinline__ void run_item(const ListPlan& P, uint32_t li, uint32_
 
 // items == nullptr: one list (`single`), item q is MARK q (q < m), MOVE q-m
 // (q < 2m) or PAIR q-2m; with m == 1 the only item is SINGLE.
-template <int WG, int IPT>
+template <int WG, int IPT, bool FUSED>
 __global__ __launch_bounds__(WG) void rz_queue(ListPlan single, const ListPlan* __restrict__ plans,
                                                 const uint2* __restrict__ items, QueueSplit qs, uint32_t chunk,
                                                 uint32_t* __restrict__ qctl, uint32_t* __restrict__ done,
                                                 uint32_t nlists, uint32_t* __restrict__ bits,
-                                                uint64_t* __restrict__ counts, uint32_t* __restrict__ ctrl) {
+                                                uint64_t* __restrict__ counts, uint64_t* __restrict__ slots,
+                                                uint32_t* __restrict__ ctrl) {
   __shared__ MoveShared<WG, IPT> msh;
   __shared__ PairShared<WG, kQueueMaxSlices> psh;
   __shared__ uint32_t s_q, s_last;
@@ -147,9 +176,12 @@ __global__ __launch_bounds__(WG) void rz_queue(ListPlan single, const ListPlan* 
             j = q - 2 * P.m;
           }
         }
-        if (P.esz == 4) run_item<uint32_t, WG, IPT>(P, li, kind, j, done, bits, counts, ctrl, msh, psh);
-        else if (P.esz == 8) run_item<uint64_t, WG, IPT>(P, li, kind, j, done, bits, counts, ctrl, msh, psh);
-        else run_item<uint4, WG, IPT>(P, li, kind, j, done, bits, counts, ctrl, msh, psh);
+        if (P.esz == 4)
+          run_item<uint32_t, WG, IPT, FUSED>(P, li, kind, j, done, bits, counts, slots, ctrl, msh, psh);
+        else if (P.esz == 8)
+          run_item<uint64_t, WG, IPT, FUSED>(P, li, kind, j, done, bits, counts, slots, ctrl, msh, psh);
+        else
+          run_item<uint4, WG, IPT, FUSED>(P, li, kind, j, done, bits, counts, slots, ctrl, msh, psh);
       }
     }
   }
@@ -172,7 +204,7 @@ __global__ __launch_bounds__(WG) void rz_queue(ListPlan single, const ListPlan* 
 // host side
 // ---------------------------------------------------------------------------
 typedef void (*QueueKernel)(ListPlan, const ListPlan*, const uint2*, QueueSplit, uint32_t, uint32_t*, uint32_t*,
-                            uint32_t, uint32_t*, uint64_t*, uint32_t*);
+                            uint32_t, uint32_t*, uint64_t*, uint64_t*, uint32_t*);
 
 // every item in queue 0 (one list, or a plan that is not split)
 static QueueSplit one_queue(uint32_t nitems) {
@@ -182,17 +214,17 @@ static QueueSplit one_queue(uint32_t nitems) {
   return q;
 }
 
-static QueueKernel queue_kernel(int wg, int ipt) {
-  if (wg == 256 && ipt == 4) return rz_queue<256, 4>;
-  if (wg == 256 && ipt == 8) return rz_queue<256, 8>;
-  if (wg == 512 && ipt == 4) return rz_queue<512, 4>;
-  if (wg == 512 && ipt == 8) return rz_queue<512, 8>;
+static QueueKernel queue_kernel(int wg, int ipt, bool fused = false) {
+  if (wg == 256 && ipt == 4) return fused ? rz_queue<256, 4, true> : rz_queue<256, 4, false>;
+  if (wg == 256 && ipt == 8) return fused ? rz_queue<256, 8, true> : rz_queue<256, 8, false>;
+  if (wg == 512 && ipt == 4) return fused ? rz_queue<512, 4, true> : rz_queue<512, 4, false>;
+  if (wg == 512 && ipt == 8) return fused ? rz_queue<512, 8, true> : rz_queue<512, 8, false>;
   return nullptr;
 }
 
-static rz_status queue_grid(rz_ctx* ctx, QueueKernel kern, int wg, uint32_t nitems, uint32_t* grid) {
+static rz_status queue_grid(rz_ctx* ctx, QueueKernel kern, int wg, size_t lds, uint32_t nitems, uint32_t* grid) {
   int per_cu = 0;
-  RZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, wg, 0));
+  RZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, wg, lds));
   if (per_cu < 1) per_cu = 1;
   uint32_t g = (uint32_t)per_cu * (uint32_t)ctx->cu_count;
   if (g > nitems) g = nitems;
@@ -244,13 +276,13 @@ rz_status launch_queue_single(rz_ctx* ctx, void* data, int64_t n, uint32_t esz, 
   if ((st = ensure(ctx, ctx->queue, 4096, s)) != RZ_OK) return st;
   const uint32_t nitems = P.m == 1 ? 1 : 2 * P.m + P.npair;
   uint32_t grid;
-  if ((st = queue_grid(ctx, kern, cfg.wg_size, nitems, &grid)) != RZ_OK) return st;
+  if ((st = queue_grid(ctx, kern, cfg.wg_size, 0, nitems, &grid)) != RZ_OK) return st;
   uint32_t* qctl = (uint32_t*)ctx->queue.ptr;
   if ((st = prof_mark(ctx, flags, 0, s)) != RZ_OK) return st;
   if ((st = prof_mark(ctx, flags, 1, s)) != RZ_OK) return st;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(cfg.wg_size), 0, s, P, (const ListPlan*)nullptr, (const uint2*)nullptr,
                      one_queue(nitems), 1u, qctl, qctl + kQWords, 1u, (uint32_t*)ctx->tail_bits.ptr,
-                     (uint64_t*)ctx->counts.ptr, (uint32_t*)ctx->ctrl.ptr);
+                     (uint64_t*)ctx->counts.ptr, (uint64_t*)nullptr, (uint32_t*)ctx->ctrl.ptr);
   RZ_HIP(hipGetLastError());
   if ((st = prof_mark(ctx, flags, 2, s)) != RZ_OK) return st;
   return prof_mark(ctx, flags, 3, s);
@@ -269,7 +301,8 @@ struct rz_batch {
   uint32_t nitems = 0, grid = 0, chunk = 1;
   rz::QueueKernel kern = nullptr;
   rz::QueueSplit split;
-  rz::DeviceBuffer plans, items, qctl, bits, counts, orphans;
+  rz::DeviceBuffer plans, items, qctl, bits, counts, orphans, slots;
+  size_t lds = 0;  // dynamic LDS of the launch: the longest fused-item slice's orphan list
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   rz_batch_info info;
@@ -308,7 +341,8 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
   plans.resize((size_t)nlists);
   items.clear();
   keys.clear();
-  uint64_t bits_words = 0, count_words = 0, orphan_words = 0;
+  uint64_t bits_words = 0, count_words = 0, orphan_words = 0, slot_words = 0;
+  uint32_t fused_L = 0;
   for (int32_t i = 0; i < nlists; ++i) {
     const rz_list_desc& d = lists[i];
     if (d.n < 0 || d.k < 0 || d.k > d.n || d.n >= (1ll << 31) || esz_index(d.elem_bytes) < 0 ||
@@ -319,6 +353,14 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
     plans[i].counts_off = (uint32_t)count_words;
     if (flags & RZ_PRESERVE_REMOVALS) plans[i].Rd = orphan_words;  // offset, patched after allocation
     if (d.k > 0 && d.k < d.n) {
+      // batch variant 4: a list of several slices short enough for LDS runs fused items
+      const uint64_t nslot = (uint64_t)d.k / 2 + 1;
+      if (cfg.variant == 4 && plans[i].m > 1 && plans[i].L <= kFusedItemMaxL && slot_words + nslot < (1ull << 32)) {
+        plans[i].npair = 0;
+        plans[i].slots_off = (uint32_t)slot_words;
+        slot_words += nslot;
+        fused_L = std::max(fused_L, plans[i].L);
+      }
       bits_words += (uint64_t)((d.k + 63) / 64) * 2;
       count_words += plans[i].m;
       orphan_words += (uint64_t)d.k;
@@ -362,7 +404,7 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
     for (size_t i = 0; i < seq.size(); ++i) {
       if (i == 0) push_all(seq[0], kMark, plans[seq[0]].m);
       if (i + 1 < seq.size()) push_all(seq[i + 1], kMark, plans[seq[i + 1]].m);
-      push_all(seq[i], kMoveItem, plans[seq[i]].m);
+      push_all(seq[i], plans[seq[i]].npair ? kMoveItem : kFusedMove, plans[seq[i]].m);
       if (i >= 1) push_all(seq[i - 1], kPair, plans[seq[i - 1]].npair);
     }
     if (!seq.empty()) push_all(seq.back(), kPair, plans[seq.back()].npair);
@@ -379,6 +421,7 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
       (st = ensure(ctx, b->qctl, (size_t)(kQWords + 2 * (size_t)nlists + 64) * 4, s)) != RZ_OK ||
       (st = ensure(ctx, b->bits, std::max<size_t>(8, (size_t)bits_words * 4), s)) != RZ_OK ||
       (st = ensure(ctx, b->counts, std::max<size_t>(8, (size_t)count_words * 8), s)) != RZ_OK ||
+      (st = ensure(ctx, b->slots, std::max<size_t>(8, (size_t)slot_words * 8), s)) != RZ_OK ||
       ((flags & RZ_PRESERVE_REMOVALS) &&
        (st = ensure(ctx, b->orphans, std::max<size_t>(8, (size_t)orphan_words * 4), s)) != RZ_OK))
     return st;
@@ -407,11 +450,14 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
     if (plan_bytes) RZ_HIP(hipMemcpy(b->plans.ptr, plans.data(), plan_bytes, hipMemcpyHostToDevice));
     if (item_bytes) RZ_HIP(hipMemcpy(b->items.ptr, items.data(), item_bytes, hipMemcpyHostToDevice));
   }
-  if (kern != b->kern) {
+  const size_t lds = (size_t)fused_L * 4;
+  if (fused_L) kern = queue_kernel(cfg.wg_size, cfg.items, true);  // some list runs fused items
+  if (kern != b->kern || lds != b->lds) {
     b->kern = kern;
+    b->lds = lds;
     b->grid_cap = 0;
   }
-  if (!b->grid_cap && (st = queue_grid(ctx, kern, cfg.wg_size, 0xffffffffu, &b->grid_cap)) != RZ_OK) return st;
+  if (!b->grid_cap && (st = queue_grid(ctx, kern, cfg.wg_size, lds, 0xffffffffu, &b->grid_cap)) != RZ_OK) return st;
   b->nlists = nlists;
   b->cfg = cfg;
   b->flags = flags;
@@ -486,10 +532,10 @@ rz_status rz_batch_launch(rz_batch* b, void* stream) {
     return RZ_OK;
   }
   uint32_t* q = (uint32_t*)b->qctl.ptr;
-  hipLaunchKernelGGL(b->kern, dim3(b->grid), dim3(b->cfg.wg_size), 0, s, ListPlan{},
+  hipLaunchKernelGGL(b->kern, dim3(b->grid), dim3(b->cfg.wg_size), b->lds, s, ListPlan{},
                      (const ListPlan*)b->plans.ptr, (const uint2*)b->items.ptr, b->split, b->chunk, q,
                      q + kQWords, (uint32_t)b->nlists, (uint32_t*)b->bits.ptr, (uint64_t*)b->counts.ptr,
-                     (uint32_t*)b->ctx->ctrl.ptr);
+                     (uint64_t*)b->slots.ptr, (uint32_t*)b->ctx->ctrl.ptr);
   RZ_HIP(hipGetLastError());
   return RZ_OK;
 }
@@ -506,10 +552,11 @@ rz_status rz_batch_capture(rz_batch* b, void* stream) {
     b->graph = nullptr;
   }
   RZ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  hipLaunchKernelGGL(b->kern, dim3(b->grid), dim3(b->cfg.wg_size), 0, s, ListPlan{},
+  hipLaunchKernelGGL(b->kern, dim3(b->grid), dim3(b->cfg.wg_size), b->lds, s, ListPlan{},
                      (const ListPlan*)b->plans.ptr, (const uint2*)b->items.ptr, b->split, b->chunk,
                      (uint32_t*)b->qctl.ptr, (uint32_t*)b->qctl.ptr + kQWords, (uint32_t)b->nlists,
-                     (uint32_t*)b->bits.ptr, (uint64_t*)b->counts.ptr, (uint32_t*)b->ctx->ctrl.ptr);
+                     (uint32_t*)b->bits.ptr, (uint64_t*)b->counts.ptr, (uint64_t*)b->slots.ptr,
+                     (uint32_t*)b->ctx->ctrl.ptr);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(s, &g);
   if (e != hipSuccess || !g) return RZ_EHIP;
@@ -540,6 +587,7 @@ rz_status rz_batch_destroy(rz_batch* b) {
   free_buf(b->bits);
   free_buf(b->counts);
   free_buf(b->orphans);
+  free_buf(b->slots);
   if (b->staging) (void)hipHostFree(b->staging);
   if (b->staged_ev) (void)hipEventDestroy(b->staged_ev);
   delete b;

@@ -101,6 +101,9 @@ static const rz_tile_config kBatchCandidates[] = {
     {256, 4, 0, 2, 1, 1}, {256, 4, 0, 2, 4, 1}, {256, 8, 0, 2, 1, 1}, {256, 8, 0, 2, 2, 1},
     {256, 8, 0, 2, 4, 1}, {512, 8, 0, 2, 1, 1}, {512, 8, 0, 2, 2, 1}, {512, 4, 0, 2, 2, 1},
     {256, 8, 0, 2, 2, 4}, {512, 8, 0, 2, 1, 2},
+    // fused items (a list's slices pair their orphans through look-back and exchange words)
+    {256, 4, 0, 4, 4, 1}, {256, 8, 0, 4, 1, 1}, {256, 8, 0, 4, 2, 1}, {512, 8, 0, 4, 1, 1},
+    {512, 8, 0, 4, 2, 1}, {512, 4, 0, 4, 2, 1}, {512, 8, 0, 4, 1, 2},
 };
 
 }  // namespace rz
@@ -137,7 +140,7 @@ rz_status rz_batch_config_get(rz_ctx* ctx, rz_tile_config* out) {
 }
 
 rz_status rz_batch_config_set(rz_ctx* ctx, const rz_tile_config* cfg) {
-  if (!ctx || !cfg || cfg->variant != 2) return RZ_EINVAL;
+  if (!ctx || !cfg || (cfg->variant != 2 && cfg->variant != 4)) return RZ_EINVAL;
   ctx->batch_cfg = *cfg;
   return RZ_OK;
 }
