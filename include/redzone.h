@@ -252,7 +252,8 @@ typedef struct rz_list_desc {
   int64_t n;
   int64_t k;
   uint32_t elem_bytes; /* 4, 8 or 16 */
-  uint32_t method;     /* 0 = Redzone (the only method the queue runs) */
+  uint32_t method;     /* RZ_METHOD_PARTITION (3): the list runs the O(n) stable partition inside the batch
+                          (survivors in their original order, as rz_partition_on); any other value: Redzone */
 } rz_list_desc;
 
 typedef struct rz_batch rz_batch;

@@ -347,8 +347,11 @@ class Context:
         return arr, len(lists)
 
     def batch(self, lists, flags=0, config=None):
-        """Plans a batch over [(data, removals), ...]; returns a Batch."""
-        arr, cnt = self._descs(lists)
+        """Plans a batch over [(data, removals), ...] or a ListDescs; returns a Batch."""
+        if isinstance(lists, ListDescs):
+            arr, cnt = lists.as_ctypes(), len(lists)
+        else:
+            arr, cnt = self._descs(lists)
         h = ctypes.c_void_p()
         cfg = ctypes.byref(config) if config is not None else None
         self._call("rz_batch_create", self._h, arr, cnt, flags, cfg, ctypes.byref(h))

@@ -16,6 +16,14 @@
 //                 rz_fused_dev.hpp).  The list's MARK j also zeroes the
 //                 slice's look-back word and its share of the list's exchange
 //                 words, so no list state survives from an earlier launch.
+// A list whose descriptor asks for the partition (method 3) runs the O(n)
+// stable compaction instead:
+//   PMARK j       marks removals [jL, (j+1)L) into the list's n-bit bitmap and
+//                 zeroes a 1/m share of the list's tile status words;
+//   PTILE j       tile j of the list's items, after every PMARK of the list
+//                 has finished (compact_tile, rz_compact_dev.hpp): tiles are
+//                 dequeued in order, so the in-place look-back pass never
+//                 overwrites an item before its tile has read it.
 // A batch is planned into one queue per XCD: lists go to XCDs by longest-
 // processing-time order on their removal counts, and all items of a list sit
 // in its XCD's queue.  Workgroup w drains queue w mod 8 first (the XCD the
@@ -38,13 +46,14 @@
 #include <algorithm>
 #include <vector>
 
+#include "rz_compact_dev.hpp"
 #include "rz_fused_dev.hpp"
 #include "rz_redzone_dev.hpp"
 
 namespace rz {
 
 constexpr int kQueueMaxSlices = 1024;  // slices per list in the queue kernel (LDS prefix)
-enum ItemKind : uint32_t { kSingle = 0, kMark = 1, kMoveItem = 2, kPair = 3, kFusedMove = 4 };
+enum ItemKind : uint32_t { kSingle = 0, kMark = 1, kMoveItem = 2, kPair = 3, kFusedMove = 4, kPMark = 5, kPTile = 6 };
 // longest slice of a list with fused items (its orphans take 4 bytes per pair of LDS)
 constexpr uint32_t kFusedItemMaxL = 8192;
 
@@ -52,11 +61,12 @@ struct ListPlan {
   uint64_t data;      // T*
   uint64_t R;         // int32_t* removal list
   uint64_t Rd;        // int32_t* dest-orphan storage (R, or preserve scratch)
-  uint32_t n, k, sZ, L;
-  uint32_t m, esz;
-  uint32_t bits_off;    // first bitmap word of this list
-  uint32_t counts_off;  // first slice-count word of this list
-  uint32_t npair;       // number of PAIR items (0: the list runs fused items)
+  uint32_t n, k, sZ, L;  // partition lists: L removals per PMARK item
+  uint32_t m;            // slices (partition lists: PMARK items)
+  uint16_t esz, method;  // method 3: the list runs PMARK and PTILE items
+  uint32_t bits_off;     // first bitmap word of this list (k bits; partition lists n bits)
+  uint32_t counts_off;   // first slice-count word of this list (partition lists: tile status words)
+  uint32_t npair;        // number of PAIR items (0: the list runs fused items; partition lists: PTILE items)
   uint32_t slots_off;   // fused items: first exchange word of this list (k/2 + 1 words)
 };
 static_assert(sizeof(ListPlan) == 64, "ListPlan is one 64-byte record");
@@ -70,9 +80,10 @@ struct QueueSplit {
   uint32_t off[kQueues + 1];
 };
 
-// FUSED: the kernel instantiation that runs fused items (batch variant 4); the
-// other one leaves their code out, which keeps its registers and occupancy.
-template <typename T, int WG, int IPT, bool FUSED>
+// FUSED / PART: the kernel instantiations that run fused items (batch variant
+// 4) / partition items; the others leave that code out, which keeps their
+// registers and occupancy.
+template <typename T, int WG, int IPT, bool FUSED, bool PART>
 __device__ __forceinline__ void run_item(const ListPlan& P, uint32_t li, uint32_t kind, uint32_t j,
                                          uint32_t* __restrict__ done, uint32_t* __restrict__ bits,
                                          uint64_t* __restrict__ counts, uint64_t* __restrict__ slots,
@@ -105,6 +116,23 @@ __device__ __forceinline__ void run_item(const ListPlan& P, uint32_t li, uint32_
       for (uint32_t w = lo + tid; w < hi; w += WG) st_relaxed(&ls[w], 0ull);
     }
     signal_done(&done[2 * li]);
+  } else if (kind == kPMark) {
+    if constexpr (PART) {
+      const uint32_t a = j * P.L, b = min(a + P.L, P.k);
+      for (uint32_t i = a + tid; i < b; i += WG) {
+        const uint32_t r = (uint32_t)R[i];
+        if (r < P.n) or_relaxed(&lb[r >> 5], 1u << (r & 31));
+      }
+      const uint32_t share = (P.npair + P.m - 1) / P.m;
+      const uint32_t lo = j * share, hi = min(lo + share, P.npair);
+      for (uint32_t w = lo + tid; w < hi; w += WG) st_relaxed(&lc[w], 0ull);
+      signal_done(&done[2 * li]);
+    }
+  } else if (kind == kPTile) {
+    if constexpr (PART) {
+      wait_count(&done[2 * li], P.m, ctrl);
+      compact_tile<T, WG, IPT, true>(A, P.n, lb, lc, j, 0u, ctrl);
+    }
   } else if (kind == kFusedMove) {
     if constexpr (FUSED) {
       wait_count(&done[2 * li], P.m, ctrl);
@@ -122,7 +150,7 @@ __device__ __forceinline__ void run_item(const ListPlan& P, uint32_t li, uint32_
 
 // items == nullptr: one list (`single`), item q is MARK q (q < m), MOVE q-m
 // (q < 2m) or PAIR q-2m; with m == 1 the only item is SINGLE.
-template <int WG, int IPT, bool FUSED>
+template <int WG, int IPT, bool FUSED, bool PART>
 __global__ __launch_bounds__(WG) void rz_queue(ListPlan single, const ListPlan* __restrict__ plans,
                                                 const uint2* __restrict__ items, QueueSplit qs, uint32_t chunk,
                                                 uint32_t* __restrict__ qctl, uint32_t* __restrict__ done,
@@ -177,11 +205,11 @@ __global__ __launch_bounds__(WG) void rz_queue(ListPlan single, const ListPlan* 
           }
         }
         if (P.esz == 4)
-          run_item<uint32_t, WG, IPT, FUSED>(P, li, kind, j, done, bits, counts, slots, ctrl, msh, psh);
+          run_item<uint32_t, WG, IPT, FUSED, PART>(P, li, kind, j, done, bits, counts, slots, ctrl, msh, psh);
         else if (P.esz == 8)
-          run_item<uint64_t, WG, IPT, FUSED>(P, li, kind, j, done, bits, counts, slots, ctrl, msh, psh);
+          run_item<uint64_t, WG, IPT, FUSED, PART>(P, li, kind, j, done, bits, counts, slots, ctrl, msh, psh);
         else
-          run_item<uint4, WG, IPT, FUSED>(P, li, kind, j, done, bits, counts, slots, ctrl, msh, psh);
+          run_item<uint4, WG, IPT, FUSED, PART>(P, li, kind, j, done, bits, counts, slots, ctrl, msh, psh);
       }
     }
   }
@@ -214,11 +242,16 @@ static QueueSplit one_queue(uint32_t nitems) {
   return q;
 }
 
-static QueueKernel queue_kernel(int wg, int ipt, bool fused = false) {
-  if (wg == 256 && ipt == 4) return fused ? rz_queue<256, 4, true> : rz_queue<256, 4, false>;
-  if (wg == 256 && ipt == 8) return fused ? rz_queue<256, 8, true> : rz_queue<256, 8, false>;
-  if (wg == 512 && ipt == 4) return fused ? rz_queue<512, 4, true> : rz_queue<512, 4, false>;
-  if (wg == 512 && ipt == 8) return fused ? rz_queue<512, 8, true> : rz_queue<512, 8, false>;
+template <int WG, int IPT>
+static QueueKernel queue_kernel_of(bool fused, bool part) {
+  if (fused) return part ? rz_queue<WG, IPT, true, true> : rz_queue<WG, IPT, true, false>;
+  return part ? rz_queue<WG, IPT, false, true> : rz_queue<WG, IPT, false, false>;
+}
+static QueueKernel queue_kernel(int wg, int ipt, bool fused = false, bool part = false) {
+  if (wg == 256 && ipt == 4) return queue_kernel_of<256, 4>(fused, part);
+  if (wg == 256 && ipt == 8) return queue_kernel_of<256, 8>(fused, part);
+  if (wg == 512 && ipt == 4) return queue_kernel_of<512, 4>(fused, part);
+  if (wg == 512 && ipt == 8) return queue_kernel_of<512, 8>(fused, part);
   return nullptr;
 }
 
@@ -343,6 +376,7 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
   keys.clear();
   uint64_t bits_words = 0, count_words = 0, orphan_words = 0, slot_words = 0;
   uint32_t fused_L = 0;
+  bool any_part = false;
   for (int32_t i = 0; i < nlists; ++i) {
     const rz_list_desc& d = lists[i];
     if (d.n < 0 || d.k < 0 || d.k > d.n || d.n >= (1ll << 31) || esz_index(d.elem_bytes) < 0 ||
@@ -352,7 +386,19 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
     plans[i].bits_off = (uint32_t)bits_words;
     plans[i].counts_off = (uint32_t)count_words;
     if (flags & RZ_PRESERVE_REMOVALS) plans[i].Rd = orphan_words;  // offset, patched after allocation
-    if (d.k > 0 && d.k < d.n) {
+    if (d.method == RZ_METHOD_PARTITION && d.k > 0 && d.k < d.n) {
+      // the O(n) stable compaction: PMARK items of 4 tiles' worth of removals, one PTILE item per tile
+      const uint32_t tile = (uint32_t)(cfg.wg_size * cfg.items);
+      ListPlan& P = plans[i];
+      P.method = RZ_METHOD_PARTITION;
+      P.L = 4 * tile;
+      P.m = (uint32_t)((d.k + P.L - 1) / P.L);
+      P.npair = (uint32_t)((d.n + tile - 1) / tile);
+      any_part = true;
+      bits_words += (uint64_t)((d.n + 63) / 64) * 2;
+      count_words += P.npair;
+      keys.push_back(((uint64_t)(0xffffffffu - (uint32_t)d.k) << 32) | (uint32_t)i);
+    } else if (d.k > 0 && d.k < d.n) {
       // batch variant 4: a list of several slices short enough for LDS runs fused items
       const uint64_t nslot = (uint64_t)d.k / 2 + 1;
       if (cfg.variant == 4 && plans[i].m > 1 && plans[i].L <= kFusedItemMaxL && slot_words + nslot < (1ull << 32)) {
@@ -374,14 +420,18 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
   // lists to XCD queues, largest first onto the least loaded queue
   std::vector<uint8_t>& xcd = b->h_xcd;
   xcd.assign((size_t)nlists, 0);
+  // queues in use: cfg.max_slices (1, 2 or 4) for batches, else one per XCD;
+  // with fewer queues the device works on fewer lists at a time
+  const int nq = (cfg.max_slices == 1 || cfg.max_slices == 2 || cfg.max_slices == 4) ? cfg.max_slices : kQueues;
   uint64_t load[kQueues] = {};
   for (uint64_t key : keys) {
     const uint32_t li = (uint32_t)key;
     int best = 0;
-    for (int x = 1; x < kQueues; ++x)
+    for (int x = 1; x < nq; ++x)
       if (load[x] < load[best]) best = x;
     xcd[li] = (uint8_t)best;
-    load[best] += (uint64_t)plans[li].k + 256u;  // removals, plus a fixed cost per list
+    // removals (a partition list: items / 16, its streaming cost on the same scale), plus a fixed cost per list
+    load[best] += (plans[li].method == RZ_METHOD_PARTITION ? (uint64_t)plans[li].n / 16 : (uint64_t)plans[li].k) + 256u;
   }
   auto push = [&](uint32_t li, uint32_t kind, uint32_t j) { items.push_back(make_uint2(li | (kind << 28), j)); };
   QueueSplit split;
@@ -396,21 +446,33 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
     seq.clear();
     for (uint64_t key : keys) {
       const uint32_t li = (uint32_t)key;
-      if (xcd[li] == x && plans[li].m > 1) seq.push_back(li);
+      if (xcd[li] == x && (plans[li].m > 1 || plans[li].method == RZ_METHOD_PARTITION)) seq.push_back(li);
     }
     auto push_all = [&](uint32_t li, uint32_t kind, uint32_t cnt) {
       for (uint32_t j = 0; j < cnt; ++j) push(li, kind, j);
     };
+    // the three phases of a list: marks, then moves (or partition tiles), then PAIR items
+    auto marks = [&](uint32_t li) {
+      push_all(li, plans[li].method == RZ_METHOD_PARTITION ? kPMark : kMark, plans[li].m);
+    };
+    auto moves = [&](uint32_t li) {
+      const ListPlan& P = plans[li];
+      if (P.method == RZ_METHOD_PARTITION) push_all(li, kPTile, P.npair);
+      else push_all(li, P.npair ? kMoveItem : kFusedMove, P.m);
+    };
+    auto pairs = [&](uint32_t li) {
+      if (plans[li].method != RZ_METHOD_PARTITION) push_all(li, kPair, plans[li].npair);
+    };
     for (size_t i = 0; i < seq.size(); ++i) {
-      if (i == 0) push_all(seq[0], kMark, plans[seq[0]].m);
-      if (i + 1 < seq.size()) push_all(seq[i + 1], kMark, plans[seq[i + 1]].m);
-      push_all(seq[i], plans[seq[i]].npair ? kMoveItem : kFusedMove, plans[seq[i]].m);
-      if (i >= 1) push_all(seq[i - 1], kPair, plans[seq[i - 1]].npair);
+      if (i == 0) marks(seq[0]);
+      if (i + 1 < seq.size()) marks(seq[i + 1]);
+      moves(seq[i]);
+      if (i >= 1) pairs(seq[i - 1]);
     }
-    if (!seq.empty()) push_all(seq.back(), kPair, plans[seq.back()].npair);
+    if (!seq.empty()) pairs(seq.back());
     for (uint64_t key : keys) {
       const uint32_t li = (uint32_t)key;
-      if (xcd[li] == x && plans[li].m == 1) push(li, kSingle, 0);
+      if (xcd[li] == x && plans[li].m == 1 && plans[li].method != RZ_METHOD_PARTITION) push(li, kSingle, 0);
     }
   }
   split.off[kQueues] = (uint32_t)items.size();
@@ -451,7 +513,8 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
     if (item_bytes) RZ_HIP(hipMemcpy(b->items.ptr, items.data(), item_bytes, hipMemcpyHostToDevice));
   }
   const size_t lds = (size_t)fused_L * 4;
-  if (fused_L) kern = queue_kernel(cfg.wg_size, cfg.items, true);  // some list runs fused items
+  // some list runs fused items, or partition items
+  if (fused_L || any_part) kern = queue_kernel(cfg.wg_size, cfg.items, fused_L > 0, any_part);
   if (kern != b->kern || lds != b->lds) {
     b->kern = kern;
     b->lds = lds;

@@ -29,6 +29,7 @@
 //                 every tile overlapping its output range has loaded.  It
 //                 stages the kept items in LDS and writes them as whole
 //                 vectors.
+#include "rz_compact_dev.hpp"
 #include "rz_ctx.hpp"
 #include "rz_device.hpp"
 
@@ -44,109 +45,11 @@ __global__ __launch_bounds__(256) void rz_mark_all(const int32_t* __restrict__ R
   if (blockIdx.x == 0 && threadIdx.x == 0) add_relaxed(&ctrl[kCtrlEpoch], 1u);
 }
 
-// status word: [63:62] flag (0 none, 1 aggregate, 2 inclusive prefix), [61:32] epoch, [31:0] count
-constexpr uint64_t kFlagAgg = 1ull, kFlagIncl = 2ull;
-__device__ __forceinline__ uint64_t pack_status(uint64_t flag, uint32_t epoch, uint32_t v) {
-  return (flag << 62) | ((uint64_t)(epoch & 0x3fffffffu) << 32) | v;
-}
-
 template <typename T, int WG, int IPT>
 __global__ __launch_bounds__(WG) void rz_compact(T* __restrict__ A, uint32_t n, uint32_t* __restrict__ bits,
                                                   uint64_t* __restrict__ status, uint32_t* __restrict__ ctrl) {
-  constexpr int NW = WG / kWave;
-  constexpr int E = IPT * NW;
-  constexpr int PER = (E + kWave - 1) / kWave;
-  __shared__ uint32_t s_off[E + 1];
-  __shared__ uint32_t s_excl;
-  const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
-  const uint32_t t = blockIdx.x;
-  const uint32_t base = t * (uint32_t)(WG * IPT);
   const uint32_t epoch = ld_relaxed(&ctrl[kCtrlEpoch]) & 0x3fffffffu;
-
-  T x[IPT];
-  uint32_t rank[IPT];
-  uint32_t keepbits = 0;
-#pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const uint32_t p = base + j * WG + tid;
-    uint32_t w = 0xffffffffu;
-    x[j] = T();  // defined in every lane: keeps 16-byte items in registers
-    if (p < n) {
-      x[j] = A[p];
-      w = bits[p >> 5];
-    }
-    const bool keep = !((w >> (p & 31)) & 1u);
-    const uint64_t m = ballot(keep);
-    rank[j] = lane_rank(m);
-    keepbits |= (uint32_t)keep << j;
-    if (lane == 0) s_off[j * NW + wave] = (uint32_t)__popcll(m);
-  }
-  wait_all_memory();  // all item loads of this tile have returned
-  __syncthreads();
-  if (wave == 0) {
-    uint32_t loc[PER];
-    uint32_t sum = 0;
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int idx = lane * PER + q;
-      loc[q] = idx < E ? s_off[idx] : 0u;
-      sum += loc[q];
-    }
-    const uint32_t xs = wave_incl_scan(sum);
-    uint32_t ex = xs - sum;
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int idx = lane * PER + q;
-      if (idx < E) s_off[idx] = ex;
-      ex += loc[q];
-    }
-    const uint32_t agg = __shfl(xs, kWave - 1, kWave);
-    // decoupled look-back
-    uint32_t excl = 0;
-    if (t == 0) {
-      if (lane == 0) st_relaxed(&status[0], pack_status(kFlagIncl, epoch, agg));
-    } else {
-      if (lane == 0) st_relaxed(&status[t], pack_status(kFlagAgg, epoch, agg));
-      int64_t j = (int64_t)t - 1;
-      uint32_t spins = 0;
-      while (true) {
-        const int64_t q = j - lane;
-        uint64_t st = q >= 0 ? ld_relaxed(&status[q]) : pack_status(kFlagIncl, epoch, 0u);
-        uint32_t flag = (((uint32_t)(st >> 32) & 0x3fffffffu) == epoch) ? (uint32_t)(st >> 62) : 0u;
-        if (q < 0) flag = (uint32_t)kFlagIncl;
-        const uint64_t incl_m = ballot(flag == kFlagIncl);
-        const uint64_t none_m = ballot(flag == 0u);
-        // lanes up to and including the nearest inclusive prefix must all be ready
-        const uint64_t upto = incl_m ? ((incl_m & (~incl_m + 1)) << 1) - 1 : ~0ull;
-        if (none_m & upto) {
-          if (++spins > spin_cap(ctrl, 1u << 24)) {  // give up (the grid is not making progress)
-            if (lane == 0) or_relaxed(&ctrl[kCtrlError], kErrLookbackTimeout);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-          continue;
-        }
-        const uint32_t v = ((upto >> lane) & 1ull) ? (uint32_t)st : 0u;
-        excl += wave_sum(v);
-        if (incl_m) break;
-        j -= kWave;
-      }
-      if (lane == 0) st_relaxed(&status[t], pack_status(kFlagIncl, epoch, excl + agg));
-    }
-    if (lane == 0) s_excl = excl;
-  }
-  __syncthreads();
-  const uint32_t excl = s_excl;
-#pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    if ((keepbits >> j) & 1u) {
-      const uint32_t pos = excl + s_off[j * NW + wave] + rank[j];
-      if (pos < n) A[pos] = x[j];
-    }
-  }
-  // clear the bitmap words this tile read (tiles are word aligned)
-  const uint32_t w0 = base >> 5, w1 = min((base + WG * IPT + 31u) >> 5, (n + 31u) >> 5);
-  for (uint32_t w = w0 + tid; w < w1; w += WG) bits[w] = 0u;
+  compact_tile<T, WG, IPT, false>(A, n, bits, status, blockIdx.x, epoch, ctrl);
 }
 
 // ---------------------------------------------------------------------------

@@ -210,16 +210,24 @@ def _side_stream(ctx):
     return st
 
 
-def run_batched_learned(ctx, disp, lists, specs=None, batch_max_k=BATCH_MAX_K):
-    """Lists below batch_max_k removals in one persistent launch (Redzone);
-    the others launched on their own with the method the MLP picks.  The
-    features and the forward pass of the large lists run on a second stream
-    while the batch runs, and only the choice comes back to the host.  (The
-    large lists' own launches were also tried on that stream, beside the
-    batch, through a second context: no gain, the two share the memory
-    system.)  Wall time (s) of everything including the decision, the
-    descriptor upload and the batch plan; returns (wall, method that ran per
-    list).  lists: [(data, removals), ...] or a ListDescs."""
+def run_batched_learned(ctx, disp, lists, specs=None, batch_max_k=BATCH_MAX_K, batch_partition=False):
+    """Lists below batch_max_k removals in one persistent launch of Redzone
+    items; the others launched on their own with the method the MLP picks.
+    The network's features and forward pass run on a second stream, for the
+    large lists beside the batch, and only the choice comes back to the host.
+    With batch_partition the network also decides for every batched list
+    before the batch is planned, and a batched list it sends to the partition
+    runs the batch's partition items (the stable O(n) compaction).  That is
+    slower on the mixed stream (measured: the batch 40.6 ms with Redzone items
+    only, 50.5 ms with the network's partition picks, and the decision for
+    every list adds 26 ms before the batch), because a partition list marks
+    all k removals and streams all n items where Redzone items mark the tail's
+    removals and write the holes; hence the default.  (The large lists' own
+    launches were also tried on the second stream, beside the batch, through a
+    second context: no gain, the two share the memory system.)  Wall time (s)
+    of everything including the decision, the descriptor upload and the batch
+    plan; returns (wall, method that ran per list).  lists: [(data,
+    removals), ...] or a ListDescs."""
     import torch
 
     from .dispatch import rule_choice
@@ -228,23 +236,32 @@ def run_batched_learned(ctx, disp, lists, specs=None, batch_max_k=BATCH_MAX_K):
     n, k = descs.rows["n"], descs.rows["k"]
     small = np.nonzero(k < batch_max_k)[0]
     large = np.nonzero(k >= batch_max_k)[0]
-    choice = np.zeros(len(descs), np.int32)  # the batch runs Redzone
+    choice = np.zeros(len(descs), np.int32)  # Redzone unless the network picks otherwise
     s = _stream_arg(ctx)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     side = _side_stream(ctx)
+    decide_small = batch_partition and disp.trained and len(small) > 0
     C = None
-    if len(large):
+    if (len(large) or decide_small) and disp.trained:
         side.wait_stream(torch.cuda.current_stream(ctx.device))  # the lists are ready on the launch stream
-    if len(large) and disp.trained:
+        which = descs if decide_small else descs.subset(large)
         with torch.cuda.stream(side):
-            _, C = disp.predict(disp.features(descs.subset(large), stream=side), stream=side)
+            _, C = disp.predict(disp.features(which, stream=side), stream=side)
     if len(small):
-        ctx.remove_batched(descs.subset(small))
+        sub = descs.subset(small)
+        if decide_small:
+            with torch.cuda.stream(side):
+                C = C.cpu().numpy()
+            sub.rows["method"] = np.where(C[small] == 3, 3, 0)
+            choice[small] = sub.rows["method"]
+        ctx.remove_batched(sub)
     if len(large):
         if C is not None:
-            with torch.cuda.stream(side):
-                choice[large] = C.cpu().numpy()
+            if not isinstance(C, np.ndarray):
+                with torch.cuda.stream(side):
+                    C = C.cpu().numpy()
+            choice[large] = C[large] if decide_small else C
         else:
             choice[large] = [rule_choice(a, b) for a, b in zip(n[large], k[large])]
         _launch_each(ctx, descs.rows[large].tolist(), [int(choice[i]) for i in large], s)

@@ -112,6 +112,34 @@ def test_fused_items_replanned_one_shot_and_preserve(ctx, oracle):
     assert ctx.check_errors() == 0
 
 
+def test_fused_items_mixed_stream_sample(ctx, oracle):
+    """The batchable lists (fewer than 2^22 removals) of a seeded sample of the
+    mixed stream (n log-uniform in [2^12, 2^26], 4/8/16-byte items, three
+    removal patterns) in fused batches, each list checked against the oracle."""
+    from redzone_amd import mixed as MX
+    from tests.parity import expected_canonical, first_mismatch
+
+    specs = [s for s in MX.stream_specs(160, seed=19) if s[1] < MX.BATCH_MAX_K]
+    assert {s[2] for s in specs} == {4, 8, 16}
+    cfg = rz.TileConfig(512, 8, 0, 4, 1, 1)
+    for g in MX.batches(specs, 4 << 30):
+        announce("fused mixed sample", g[0], g[-1])
+        lists = [MX.build_list(ctx, specs[i], 9100 + 7 * i) for i in g]
+        host = [(A.cpu().numpy(), R.cpu().numpy()) for A, R in lists]
+        b = ctx.batch(lists, config=cfg)
+        b.launch()
+        torch.cuda.synchronize()
+        b.close()
+        assert ctx.check_errors() == 0
+        for j, i in enumerate(g):
+            n, k = specs[i][0], specs[i][1]
+            want = expected_canonical(oracle, host[j][0], host[j][1])
+            m = first_mismatch(lists[j][0].cpu().numpy()[: n - k], want[: n - k])
+            assert m < 0, (specs[i], m)
+        del lists, host
+        torch.cuda.empty_cache()
+
+
 def test_fused_items_orphan_mismatch(ctx, oracle):
     """A list that breaks the contract (its last index listed twice) beside
     valid ones: the fused items see the orphan totals differ and the check

@@ -43,8 +43,8 @@ def test_batched_learned_path_matches_oracle(ctx, oracle):
     for i, ((A0, R0), (A, _)) in enumerate(zip(host, lists)):
         n, k = specs[i][0], specs[i][1]
         got = A.cpu().numpy()
-        # lists below the batch threshold run in the (Redzone) batch whatever the choice
-        if choice[i] == 3 and k >= bmk:
+        # the choice is the method that ran: the partition (in the batch or on its own) or Redzone
+        if choice[i] == 3:
             bad, fb = oracle.check(A0, got, R0, mode=1)
             assert bad == 0, (i, specs[i], bad, fb)
         else:
@@ -129,25 +129,27 @@ def _config5_sample():
 def test_config5_sample_batched_learned(ctx, oracle):
     """Every list of the sample through run_batched_learned (the batch below
     2^22 removals, the network's method above), in resident groups, each list
-    checked against the oracle."""
+    checked against the oracle; then the same with the network's partition
+    picks run as the batch's partition items."""
     disp = D.Dispatcher.load(ctx)
     specs = _config5_sample()
     assert {s[2] for s in specs} == {4, 8, 16} and max(s[0] for s in specs) >= (1 << 26) - 1
     ran = {0: 0, 1: 0, 2: 0, 3: 0}
     for g in MX.batches(specs, 6 << 30):
         announce("config5 group", g[0], g[-1])
-        lists = [MX.build_list(ctx, specs[i], 5000 + 7 * i) for i in g]
-        host = [(A.cpu().numpy(), R.cpu().numpy()) for A, R in lists]
-        _, choice = MX.run_batched_learned(ctx, disp, lists)
-        torch.cuda.synchronize()
-        assert ctx.check_errors() == 0
-        for j, i in enumerate(g):
-            k = specs[i][1]
-            m = int(choice[j]) if k >= MX.BATCH_MAX_K else 0  # the batch runs Redzone
-            ran[m] += 1
-            _check_list(oracle, specs[i], host[j][0], host[j][1], lists[j][0].cpu().numpy(), m == 3)
-        del lists, host
-        torch.cuda.empty_cache()
+        for batch_partition in (False, True):
+            lists = [MX.build_list(ctx, specs[i], 5000 + 7 * i) for i in g]
+            host = [(A.cpu().numpy(), R.cpu().numpy()) for A, R in lists]
+            _, choice = MX.run_batched_learned(ctx, disp, lists, batch_partition=batch_partition)
+            torch.cuda.synchronize()
+            assert ctx.check_errors() == 0
+            for j, i in enumerate(g):
+                m = int(choice[j])  # the method that ran (batched lists: Redzone, or partition items)
+                if not batch_partition:
+                    ran[m] += 1
+                _check_list(oracle, specs[i], host[j][0], host[j][1], lists[j][0].cpu().numpy(), m == 3)
+            del lists, host
+            torch.cuda.empty_cache()
     assert ran[0] > 100 and sum(ran.values()) == len(specs)
 
 
