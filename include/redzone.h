@@ -73,7 +73,8 @@ typedef struct rz_tile_config {
   int32_t wg_size;     /* 256 or 512 */
   int32_t items;       /* items per thread per step: 2, 4, 8, 16 (variants 0/1); 4 or 8 (variant 2, batches) */
   int32_t max_slices;  /* variants 0/1/3/4/6: upper bound on slices (workgroups of the move step);
-                          batches: queues in use, 1, 2 or 4 (any other value: one per XCD, 8) */
+                          batches: queues in use, 1, 2 or 4 (any other value: one per XCD, 8); 16: 8 queues
+                          with every list's items dealt over all of them */
   int32_t variant;     /* 0 = three kernels on the stream (mark, move, pair), 1 = one cooperative kernel,
                           2 = persistent queue, 3 = one ordered launch (roles by workgroup id),
                           (batches take 2 = MARK, MOVE and PAIR items per list, or 4 = MARK and fused

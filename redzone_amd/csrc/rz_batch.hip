@@ -421,8 +421,15 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
   std::vector<uint8_t>& xcd = b->h_xcd;
   xcd.assign((size_t)nlists, 0);
   // queues in use: cfg.max_slices (1, 2 or 4) for batches, else one per XCD;
-  // with fewer queues the device works on fewer lists at a time
+  // with fewer queues the device works on fewer lists at a time.  With
+  // max_slices 16 every list of several slices has its items dealt over all 8
+  // queues (item j of list l to queue (j + l) mod 8), in the same list order
+  // in each, so that all XCDs work on the same list; the one-slice lists are
+  // still assigned whole.  Each queue's order is then a subsequence of one
+  // order of all items, in which an item's dependencies all come earlier, and
+  // a workgroup leaves its home queue only once it is empty: the queues drain.
   const int nq = (cfg.max_slices == 1 || cfg.max_slices == 2 || cfg.max_slices == 4) ? cfg.max_slices : kQueues;
+  const bool spread = cfg.max_slices == 16;
   uint64_t load[kQueues] = {};
   for (uint64_t key : keys) {
     const uint32_t li = (uint32_t)key;
@@ -446,10 +453,11 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
     seq.clear();
     for (uint64_t key : keys) {
       const uint32_t li = (uint32_t)key;
-      if (xcd[li] == x && (plans[li].m > 1 || plans[li].method == RZ_METHOD_PARTITION)) seq.push_back(li);
+      if ((spread || xcd[li] == x) && (plans[li].m > 1 || plans[li].method == RZ_METHOD_PARTITION)) seq.push_back(li);
     }
     auto push_all = [&](uint32_t li, uint32_t kind, uint32_t cnt) {
-      for (uint32_t j = 0; j < cnt; ++j) push(li, kind, j);
+      for (uint32_t j = 0; j < cnt; ++j)
+        if (!spread || (j + li) % kQueues == (uint32_t)x) push(li, kind, j);
     };
     // the three phases of a list: marks, then moves (or partition tiles), then PAIR items
     auto marks = [&](uint32_t li) {

@@ -171,3 +171,25 @@ def test_remove_batched_one_shot_and_preserve(ctx, oracle):
     for d, h in zip(L.dev_R, L.host_R):
         assert torch.equal(d.cpu(), h)
     b.close()
+
+
+def test_batch_spread_queues(ctx, oracle):
+    """Batch config max_slices 16: the items of every list of several slices
+    are dealt over all eight queues (every XCD on the same list), with
+    MOVE/PAIR items, fused items and a one-queue config beside it; small lists,
+    large lists and a relaunch, each list against the oracle."""
+    rng = np.random.default_rng(1616)
+    specs = _random_specs(rng, 200, 80000)
+    specs += [(1 << 22, 1 << 21, 4, "uniform"), ((1 << 21) + 7, 900_000, 16, "clustered"),
+              (1 << 20, 300_000, 8, "tail_heavy")]
+    L = Lists(specs, seed=1700)
+    for cfg in (rz.TileConfig(512, 8, 16, 2, 1, 1), rz.TileConfig(256, 4, 16, 2, 2, 1),
+                rz.TileConfig(512, 8, 16, 4, 2, 1), rz.TileConfig(256, 8, 1, 2, 1, 1)):
+        b = ctx.batch(L.pairs(), config=cfg)
+        for rep in range(2):
+            announce("batch spread", cfg.as_tuple(), rep)
+            L.restore()
+            b.launch()
+            L.check(oracle, f"spread {cfg.as_tuple()} launch {rep}")
+        b.close()
+    assert ctx.check_errors() == 0
