@@ -77,9 +77,11 @@ typedef struct rz_tile_config {
                           with every list's items dealt over all of them */
   int32_t variant;     /* 0 = three kernels on the stream (mark, move, pair), 1 = one cooperative kernel,
                           2 = persistent queue, 3 = one ordered launch (roles by workgroup id),
-                          (batches take 2 = MARK, MOVE and PAIR items per list, or 4 = MARK and fused
+                          (batches take 2 = MARK, MOVE and PAIR items per list, 4 = MARK and fused
                           items: a list of several slices of at most 8,192 pairs each pairs its orphans
-                          inside the move items, through look-back and exchange words),
+                          inside the move items, through look-back and exchange words, or 5 = as 2, with
+                          the tail marks of lists of 2^19 to 2^22 removals grouped in buckets of 2^15
+                          tail positions and each bucket's bitmap words written whole),
                           4 = two kernels (mark, then move with in-kernel orphan pairing; slices longer than
                               the workgroup LDS allows run as variant 0),
                           5 = dense form: head writes grouped by destination bin, then one scatter per bin

@@ -16,6 +16,16 @@
 //                 rz_fused_dev.hpp).  The list's MARK j also zeroes the
 //                 slice's look-back word and its share of the list's exchange
 //                 words, so no list state survives from an earlier launch.
+// With batch variant 5, a list of 2^19 to 2^22 removals marks its tail in
+// buckets instead of one atomic per tail removal:
+//   MARK j        groups the tail removals of removals [jL, (j+1)L) by bucket
+//                 (2^15 tail positions) in LDS, reserves room in each
+//                 bucket's entry list with one atomic per bucket, and writes
+//                 the entries out bucket by bucket (whole runs, not one
+//                 scattered request per removal);
+//   BUCKET b      after every MARK of the list, sets bucket b's bits in an
+//                 LDS bitmap and writes its bitmap words whole; MOVE items wait
+//                 for the BUCKET items.
 // A list whose descriptor asks for the partition (method 3) runs the O(n)
 // stable compaction instead:
 //   PMARK j       marks removals [jL, (j+1)L) into the list's n-bit bitmap and
@@ -53,7 +63,17 @@
 namespace rz {
 
 constexpr int kQueueMaxSlices = 1024;  // slices per list in the queue kernel (LDS prefix)
-enum ItemKind : uint32_t { kSingle = 0, kMark = 1, kMoveItem = 2, kPair = 3, kFusedMove = 4, kPMark = 5, kPTile = 6 };
+enum ItemKind : uint32_t {
+  kSingle = 0, kMark = 1, kMoveItem = 2, kPair = 3, kFusedMove = 4, kPMark = 5, kPTile = 6, kBucket = 7
+};
+// the item families a queue-kernel instantiation carries besides MARK, MOVE, PAIR and SINGLE
+enum QueueMode : int { kModeFused = 1, kModePart = 2, kModeBuck = 4 };
+// bucketed tail marks (batch variant 5)
+constexpr uint32_t kBucketLog = 15, kBucketW = 1u << kBucketLog;  // tail positions per bucket (4 KB of bitmap)
+constexpr uint32_t kMaxBuckets = 128;                             // lists of fewer than 2^22 removals
+constexpr uint32_t kBucketMinK = 1u << 19;                        // ... and at least this many
+constexpr uint32_t kBucketMaxL = 8192;                            // slice length the LDS staging holds
+constexpr uint16_t kPlanBucketed = 5;                             // ListPlan::method of such a list
 // longest slice of a list with fused items (its orphans take 4 bytes per pair of LDS)
 constexpr uint32_t kFusedItemMaxL = 8192;
 
@@ -68,8 +88,10 @@ struct ListPlan {
   uint32_t counts_off;   // first slice-count word of this list (partition lists: tile status words)
   uint32_t npair;        // number of PAIR items (0: the list runs fused items; partition lists: PTILE items)
   uint32_t slots_off;   // fused items: first exchange word of this list (k/2 + 1 words)
+  uint64_t ent;          // bucketed marks: uint32_t* entry lists, kBucketW words per bucket
+  uint64_t cur;          // bucketed marks: uint32_t* entry counts, one per bucket (zero between launches)
 };
-static_assert(sizeof(ListPlan) == 64, "ListPlan is one 64-byte record");
+static_assert(sizeof(ListPlan) == 80, "ListPlan is one 80-byte record");
 
 // queue control words (uint32 index into the queue control block): the ticket
 // of queue x at kQTicket + 32 x (a line each), then the exit count
@@ -80,15 +102,93 @@ struct QueueSplit {
   uint32_t off[kQueues + 1];
 };
 
-// FUSED / PART: the kernel instantiations that run fused items (batch variant
-// 4) / partition items; the others leave that code out, which keeps their
-// registers and occupancy.
-template <typename T, int WG, int IPT, bool FUSED, bool PART>
+// Bucketed marks of removals [a, b) of a list (MARK item of batch variant 5):
+// the tail removals' positions z, grouped by bucket z >> kBucketLog in LDS,
+// appended to their buckets' entry lists in whole runs.
+template <int WG>
+__device__ __forceinline__ void bucket_marks(const int32_t* __restrict__ R, uint32_t a, uint32_t b, uint32_t k,
+                                             uint32_t sZ, uint32_t* __restrict__ ent, uint32_t* __restrict__ cur) {
+  __shared__ uint32_t s_stage[kBucketMaxL];
+  __shared__ uint32_t s_cnt[kMaxBuckets], s_lbase[kMaxBuckets], s_gbase[kMaxBuckets];
+  __shared__ uint32_t s_total;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nb = (k + kBucketW - 1) >> kBucketLog;
+  for (uint32_t w = tid; w < nb; w += WG) s_cnt[w] = 0u;
+  __syncthreads();
+  for (uint32_t i = a + tid; i < b; i += WG) {
+    const uint32_t z = (uint32_t)R[i] - sZ;  // wraps to >= 2^31 > k below the tail
+    if (z < k) atomicAdd(&s_cnt[z >> kBucketLog], 1u);
+  }
+  __syncthreads();
+  if (tid < kWave) {  // exclusive prefix over the buckets, two per lane
+    const uint32_t b0 = 2 * tid, b1 = b0 + 1;
+    const uint32_t c0 = b0 < nb ? s_cnt[b0] : 0u, c1 = b1 < nb ? s_cnt[b1] : 0u;
+    const uint32_t incl = wave_incl_scan(c0 + c1);
+    const uint32_t ex = incl - c0 - c1;
+    if (b0 < nb) s_lbase[b0] = ex;
+    if (b1 < nb) s_lbase[b1] = ex + c0;
+    if (tid == kWave - 1) s_total = incl;
+  }
+  __syncthreads();
+  for (uint32_t w = tid; w < nb; w += WG) {  // room in each bucket's list: one atomic per bucket
+    const uint32_t c = s_cnt[w];
+    s_gbase[w] = c ? add_relaxed(&cur[w], c) : 0u;
+    s_cnt[w] = 0u;
+  }
+  __syncthreads();
+  for (uint32_t i = a + tid; i < b; i += WG) {  // the entries in LDS, bucket after bucket
+    const uint32_t z = (uint32_t)R[i] - sZ;
+    if (z < k) {
+      const uint32_t bz = z >> kBucketLog;
+      s_stage[s_lbase[bz] + atomicAdd(&s_cnt[bz], 1u)] = z;
+    }
+  }
+  __syncthreads();
+  const uint32_t total = s_total;
+  for (uint32_t t = tid; t < total; t += WG) {  // consecutive threads write consecutive entries of a bucket
+    const uint32_t z = s_stage[t];
+    const uint32_t bz = z >> kBucketLog;
+    const uint32_t g = s_gbase[bz] + (t - s_lbase[bz]);
+    if (g < kBucketW) st_relaxed(&ent[(size_t)bz * kBucketW + g], z);  // more only with duplicate removals
+  }
+}
+
+// BUCKET item b: bucket b's bits into LDS, then its bitmap words written whole
+// (the bitmap is zero outside the marks); the bucket's count back to zero.
+template <int WG>
+__device__ __forceinline__ void bucket_bits(uint32_t b, uint32_t k, const uint32_t* __restrict__ ent,
+                                            uint32_t* __restrict__ cur, uint32_t* __restrict__ bits) {
+  constexpr uint32_t WORDS = kBucketW / 32;
+  __shared__ uint32_t s_bm[WORDS];
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t w = tid; w < WORDS; w += WG) s_bm[w] = 0u;
+  __syncthreads();
+  const uint32_t c = min(ld_relaxed(&cur[b]), kBucketW);
+  const uint32_t* e = ent + (size_t)b * kBucketW;
+  const uint32_t lo = b * kBucketW;
+  for (uint32_t t = tid; t < c; t += WG) {
+    const uint32_t off = ld_relaxed(&e[t]) - lo;
+    if (off < kBucketW) atomicOr(&s_bm[off >> 5], 1u << (off & 31));
+  }
+  __syncthreads();
+  const uint32_t w0 = lo >> 5, wend = (k + 31) >> 5;
+  for (uint32_t w = tid; w < WORDS; w += WG) {
+    const uint32_t v = s_bm[w];
+    if (v && w0 + w < wend) st_relaxed(&bits[w0 + w], v);
+  }
+  if (tid == 0) st_relaxed(&cur[b], 0u);
+}
+
+// MODE (kModeFused, kModePart, kModeBuck): the item families this kernel
+// instantiation runs besides MARK, MOVE, PAIR and SINGLE; the others leave that
+// code out, which keeps their registers, LDS and occupancy.
+template <typename T, int WG, int IPT, int MODE>
 __device__ __forceinline__ void run_item(const ListPlan& P, uint32_t li, uint32_t kind, uint32_t j,
                                          uint32_t* __restrict__ done, uint32_t* __restrict__ bits,
                                          uint64_t* __restrict__ counts, uint64_t* __restrict__ slots,
                                          uint32_t* __restrict__ ctrl, MoveShared<WG, IPT>& msh,
                                          PairShared<WG, kQueueMaxSlices>& psh) {
+  constexpr bool FUSED = (MODE & kModeFused) != 0, PART = (MODE & kModePart) != 0, BUCK = (MODE & kModeBuck) != 0;
   T* A = (T*)P.data;
   const int32_t* R = (const int32_t*)P.R;
   int32_t* Rd = (int32_t*)P.Rd;
@@ -105,7 +205,11 @@ __device__ __forceinline__ void run_item(const ListPlan& P, uint32_t li, uint32_
     pair_body<T, WG, kQueueMaxSlices, true>(A, Rd, P.k, P.sZ, P.L, 1, lc, ctrl, 0, 0, 1, psh);
   } else if (kind == kMark) {
     const uint32_t a = j * P.L, b = min(a + P.L, P.k);
-    mark_body<false>(R + a, b - a, P.k, P.sZ, lb, nullptr, tid, WG);
+    if (BUCK && P.method == kPlanBucketed) {
+      if constexpr (BUCK) bucket_marks<WG>(R, a, b, P.k, P.sZ, (uint32_t*)P.ent, (uint32_t*)P.cur);
+    } else {
+      mark_body<false>(R + a, b - a, P.k, P.sZ, lb, nullptr, tid, WG);
+    }
     if (FUSED && P.npair == 0) {
       // fused items: this slice's look-back word and a 1/m share of the
       // list's exchange words start at zero
@@ -115,7 +219,7 @@ __device__ __forceinline__ void run_item(const ListPlan& P, uint32_t li, uint32_
       const uint32_t lo = j * share, hi = min(lo + share, nslot);
       for (uint32_t w = lo + tid; w < hi; w += WG) st_relaxed(&ls[w], 0ull);
     }
-    signal_done(&done[2 * li]);
+    signal_done(&done[3 * li]);
   } else if (kind == kPMark) {
     if constexpr (PART) {
       const uint32_t a = j * P.L, b = min(a + P.L, P.k);
@@ -126,31 +230,40 @@ __device__ __forceinline__ void run_item(const ListPlan& P, uint32_t li, uint32_
       const uint32_t share = (P.npair + P.m - 1) / P.m;
       const uint32_t lo = j * share, hi = min(lo + share, P.npair);
       for (uint32_t w = lo + tid; w < hi; w += WG) st_relaxed(&lc[w], 0ull);
-      signal_done(&done[2 * li]);
+      signal_done(&done[3 * li]);
     }
   } else if (kind == kPTile) {
     if constexpr (PART) {
-      wait_count(&done[2 * li], P.m, ctrl);
+      wait_count(&done[3 * li], P.m, ctrl);
       compact_tile<T, WG, IPT, true>(A, P.n, lb, lc, j, 0u, ctrl);
     }
   } else if (kind == kFusedMove) {
     if constexpr (FUSED) {
-      wait_count(&done[2 * li], P.m, ctrl);
+      wait_count(&done[3 * li], P.m, ctrl);
       move_pair_core<T, WG, IPT, true, false>(A, R, P.k, P.sZ, P.L, P.m, lb, lc, slots + P.slots_off,
                                               P.k / 2 + 1, ctrl, 0, 0, j, 0u, 1u, nullptr);
     }
+  } else if (kind == kBucket) {
+    if constexpr (BUCK) {
+      wait_count(&done[3 * li], P.m, ctrl);
+      bucket_bits<WG>(j, P.k, (const uint32_t*)P.ent, (uint32_t*)P.cur, lb);
+      signal_done(&done[3 * li + 2]);
+    }
   } else if (kind == kMoveItem) {
-    wait_count(&done[2 * li], P.m, ctrl);
-    move_slice<T, WG, IPT, true>(A, R, Rd, P.k, P.sZ, P.L, j, lb, lc, msh, &done[2 * li + 1]);
+    if (BUCK && P.method == kPlanBucketed)
+      wait_count(&done[3 * li + 2], (P.k + kBucketW - 1) >> kBucketLog, ctrl);  // the BUCKET items
+    else
+      wait_count(&done[3 * li], P.m, ctrl);
+    move_slice<T, WG, IPT, true>(A, R, Rd, P.k, P.sZ, P.L, j, lb, lc, msh, &done[3 * li + 1]);
   } else {
-    wait_count(&done[2 * li + 1], P.m, ctrl);
+    wait_count(&done[3 * li + 1], P.m, ctrl);
     pair_body<T, WG, kQueueMaxSlices, true>(A, Rd, P.k, P.sZ, P.L, P.m, lc, ctrl, 0, j, P.npair, psh);
   }
 }
 
 // items == nullptr: one list (`single`), item q is MARK q (q < m), MOVE q-m
 // (q < 2m) or PAIR q-2m; with m == 1 the only item is SINGLE.
-template <int WG, int IPT, bool FUSED, bool PART>
+template <int WG, int IPT, int MODE>
 __global__ __launch_bounds__(WG) void rz_queue(ListPlan single, const ListPlan* __restrict__ plans,
                                                 const uint2* __restrict__ items, QueueSplit qs, uint32_t chunk,
                                                 uint32_t* __restrict__ qctl, uint32_t* __restrict__ done,
@@ -205,11 +318,11 @@ __global__ __launch_bounds__(WG) void rz_queue(ListPlan single, const ListPlan* 
           }
         }
         if (P.esz == 4)
-          run_item<uint32_t, WG, IPT, FUSED, PART>(P, li, kind, j, done, bits, counts, slots, ctrl, msh, psh);
+          run_item<uint32_t, WG, IPT, MODE>(P, li, kind, j, done, bits, counts, slots, ctrl, msh, psh);
         else if (P.esz == 8)
-          run_item<uint64_t, WG, IPT, FUSED, PART>(P, li, kind, j, done, bits, counts, slots, ctrl, msh, psh);
+          run_item<uint64_t, WG, IPT, MODE>(P, li, kind, j, done, bits, counts, slots, ctrl, msh, psh);
         else
-          run_item<uint4, WG, IPT, FUSED, PART>(P, li, kind, j, done, bits, counts, slots, ctrl, msh, psh);
+          run_item<uint4, WG, IPT, MODE>(P, li, kind, j, done, bits, counts, slots, ctrl, msh, psh);
       }
     }
   }
@@ -220,7 +333,7 @@ __global__ __launch_bounds__(WG) void rz_queue(ListPlan single, const ListPlan* 
   }
   __syncthreads();
   if (s_last) {
-    for (uint32_t i = tid; i < 2 * nlists; i += WG) st_relaxed(&done[i], 0u);
+    for (uint32_t i = tid; i < 3 * nlists; i += WG) st_relaxed(&done[i], 0u);
     if (tid == 0) {
       for (int x = 0; x < kQueues; ++x) st_relaxed(&qctl[kQTicket + 32 * x], 0u);
       st_relaxed(&qctl[kQExit], 0u);
@@ -243,15 +356,23 @@ static QueueSplit one_queue(uint32_t nitems) {
 }
 
 template <int WG, int IPT>
-static QueueKernel queue_kernel_of(bool fused, bool part) {
-  if (fused) return part ? rz_queue<WG, IPT, true, true> : rz_queue<WG, IPT, true, false>;
-  return part ? rz_queue<WG, IPT, false, true> : rz_queue<WG, IPT, false, false>;
+static QueueKernel queue_kernel_of(int mode) {
+  switch (mode & 7) {
+    case 0: return rz_queue<WG, IPT, 0>;
+    case 1: return rz_queue<WG, IPT, 1>;
+    case 2: return rz_queue<WG, IPT, 2>;
+    case 3: return rz_queue<WG, IPT, 3>;
+    case 4: return rz_queue<WG, IPT, 4>;
+    case 5: return rz_queue<WG, IPT, 5>;
+    case 6: return rz_queue<WG, IPT, 6>;
+    default: return rz_queue<WG, IPT, 7>;
+  }
 }
-static QueueKernel queue_kernel(int wg, int ipt, bool fused = false, bool part = false) {
-  if (wg == 256 && ipt == 4) return queue_kernel_of<256, 4>(fused, part);
-  if (wg == 256 && ipt == 8) return queue_kernel_of<256, 8>(fused, part);
-  if (wg == 512 && ipt == 4) return queue_kernel_of<512, 4>(fused, part);
-  if (wg == 512 && ipt == 8) return queue_kernel_of<512, 8>(fused, part);
+static QueueKernel queue_kernel(int wg, int ipt, int mode = 0) {
+  if (wg == 256 && ipt == 4) return queue_kernel_of<256, 4>(mode);
+  if (wg == 256 && ipt == 8) return queue_kernel_of<256, 8>(mode);
+  if (wg == 512 && ipt == 4) return queue_kernel_of<512, 4>(mode);
+  if (wg == 512 && ipt == 8) return queue_kernel_of<512, 8>(mode);
   return nullptr;
 }
 
@@ -334,7 +455,7 @@ struct rz_batch {
   uint32_t nitems = 0, grid = 0, chunk = 1;
   rz::QueueKernel kern = nullptr;
   rz::QueueSplit split;
-  rz::DeviceBuffer plans, items, qctl, bits, counts, orphans, slots;
+  rz::DeviceBuffer plans, items, qctl, bits, counts, orphans, slots, ents, curs;
   size_t lds = 0;  // dynamic LDS of the launch: the longest fused-item slice's orphan list
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -374,7 +495,7 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
   plans.resize((size_t)nlists);
   items.clear();
   keys.clear();
-  uint64_t bits_words = 0, count_words = 0, orphan_words = 0, slot_words = 0;
+  uint64_t bits_words = 0, count_words = 0, orphan_words = 0, slot_words = 0, ent_words = 0, cur_words = 0;
   uint32_t fused_L = 0;
   bool any_part = false;
   for (int32_t i = 0; i < nlists; ++i) {
@@ -406,6 +527,15 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
         plans[i].slots_off = (uint32_t)slot_words;
         slot_words += nslot;
         fused_L = std::max(fused_L, plans[i].L);
+      } else if (cfg.variant == 5 && plans[i].m > 1 && d.k >= kBucketMinK && d.k < (1ll << 22) &&
+                 plans[i].L <= kBucketMaxL) {
+        // batch variant 5: bucketed tail marks (entry lists and counts, offsets patched after allocation)
+        const uint32_t nb = (uint32_t)((d.k + kBucketW - 1) >> kBucketLog);
+        plans[i].method = kPlanBucketed;
+        plans[i].ent = ent_words;
+        plans[i].cur = cur_words;
+        ent_words += (uint64_t)nb * kBucketW;
+        cur_words += nb;
       }
       bits_words += (uint64_t)((d.k + 63) / 64) * 2;
       count_words += plans[i].m;
@@ -465,6 +595,7 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
     };
     auto moves = [&](uint32_t li) {
       const ListPlan& P = plans[li];
+      if (P.method == kPlanBucketed) push_all(li, kBucket, (P.k + kBucketW - 1) >> kBucketLog);
       if (P.method == RZ_METHOD_PARTITION) push_all(li, kPTile, P.npair);
       else push_all(li, P.npair ? kMoveItem : kFusedMove, P.m);
     };
@@ -488,7 +619,9 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
   const size_t plan_bytes = plans.size() * sizeof(ListPlan), item_bytes = items.size() * sizeof(uint2);
   if ((st = ensure(ctx, b->plans, std::max<size_t>(64, plan_bytes), s)) != RZ_OK ||
       (st = ensure(ctx, b->items, std::max<size_t>(8, item_bytes), s)) != RZ_OK ||
-      (st = ensure(ctx, b->qctl, (size_t)(kQWords + 2 * (size_t)nlists + 64) * 4, s)) != RZ_OK ||
+      (st = ensure(ctx, b->qctl, (size_t)(kQWords + 3 * (size_t)nlists + 64) * 4, s)) != RZ_OK ||
+      (st = ensure(ctx, b->ents, std::max<size_t>(8, (size_t)ent_words * 4), s)) != RZ_OK ||
+      (st = ensure(ctx, b->curs, std::max<size_t>(8, (size_t)cur_words * 4), s)) != RZ_OK ||
       (st = ensure(ctx, b->bits, std::max<size_t>(8, (size_t)bits_words * 4), s)) != RZ_OK ||
       (st = ensure(ctx, b->counts, std::max<size_t>(8, (size_t)count_words * 8), s)) != RZ_OK ||
       (st = ensure(ctx, b->slots, std::max<size_t>(8, (size_t)slot_words * 8), s)) != RZ_OK ||
@@ -497,6 +630,11 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
     return st;
   if (flags & RZ_PRESERVE_REMOVALS)
     for (auto& P : plans) P.Rd = (uint64_t)(uintptr_t)((int32_t*)b->orphans.ptr + P.Rd);
+  for (auto& P : plans)
+    if (P.method == kPlanBucketed) {
+      P.ent = (uint64_t)(uintptr_t)((uint32_t*)b->ents.ptr + P.ent);
+      P.cur = (uint64_t)(uintptr_t)((uint32_t*)b->curs.ptr + P.cur);
+    }
   if (async) {
     const size_t need = plan_bytes + item_bytes;
     if (b->staged) RZ_HIP(hipEventSynchronize(b->staged_ev));  // the previous upload has left the staging area
@@ -522,7 +660,8 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
   }
   const size_t lds = (size_t)fused_L * 4;
   // some list runs fused items, or partition items
-  if (fused_L || any_part) kern = queue_kernel(cfg.wg_size, cfg.items, fused_L > 0, any_part);
+  const int mode = (fused_L ? kModeFused : 0) | (any_part ? kModePart : 0) | (ent_words ? kModeBuck : 0);
+  if (mode) kern = queue_kernel(cfg.wg_size, cfg.items, mode);
   if (kern != b->kern || lds != b->lds) {
     b->kern = kern;
     b->lds = lds;
@@ -546,7 +685,7 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
 // After a failed device check: the batch's bitmap, slice counts and queue
 // words back to zero (the launch keeps them zero when every wait succeeds).
 rz_status rz::batch_reset_scratch(rz_batch* b) {
-  DeviceBuffer* bufs[] = {&b->qctl, &b->bits, &b->counts};
+  DeviceBuffer* bufs[] = {&b->qctl, &b->bits, &b->counts, &b->curs};
   for (DeviceBuffer* d : bufs)
     if (d->ptr) RZ_HIP(hipMemset(d->ptr, 0, d->bytes));
   return RZ_OK;
@@ -659,6 +798,8 @@ rz_status rz_batch_destroy(rz_batch* b) {
   free_buf(b->counts);
   free_buf(b->orphans);
   free_buf(b->slots);
+  free_buf(b->ents);
+  free_buf(b->curs);
   if (b->staging) (void)hipHostFree(b->staging);
   if (b->staged_ev) (void)hipEventDestroy(b->staged_ev);
   delete b;

@@ -104,6 +104,8 @@ static const rz_tile_config kBatchCandidates[] = {
     // fused items (a list's slices pair their orphans through look-back and exchange words)
     {256, 4, 0, 4, 4, 1}, {256, 8, 0, 4, 1, 1}, {256, 8, 0, 4, 2, 1}, {512, 8, 0, 4, 1, 1},
     {512, 8, 0, 4, 2, 1}, {512, 4, 0, 4, 2, 1}, {512, 8, 0, 4, 1, 2},
+    // bucketed tail marks for lists of 2^17 removals and more
+    {512, 8, 0, 5, 1, 1}, {512, 8, 0, 5, 2, 1}, {256, 8, 0, 5, 1, 1}, {512, 4, 0, 5, 2, 1},
 };
 
 }  // namespace rz
@@ -140,7 +142,7 @@ rz_status rz_batch_config_get(rz_ctx* ctx, rz_tile_config* out) {
 }
 
 rz_status rz_batch_config_set(rz_ctx* ctx, const rz_tile_config* cfg) {
-  if (!ctx || !cfg || (cfg->variant != 2 && cfg->variant != 4)) return RZ_EINVAL;
+  if (!ctx || !cfg || (cfg->variant != 2 && cfg->variant != 4 && cfg->variant != 5)) return RZ_EINVAL;
   ctx->batch_cfg = *cfg;
   return RZ_OK;
 }

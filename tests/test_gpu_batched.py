@@ -193,3 +193,71 @@ def test_batch_spread_queues(ctx, oracle):
             L.check(oracle, f"spread {cfg.as_tuple()} launch {rep}")
         b.close()
     assert ctx.check_errors() == 0
+
+
+def test_batch_bucketed_marks(ctx, oracle):
+    """Batch variant 5: lists of 2^19 to 2^22 removals mark their tails in
+    buckets (MARK items append entry runs, BUCKET items write bitmap words
+    whole); smaller lists keep the atomic marks in the same batch.  First
+    launches, a relaunch, a graph replay, RZ_PRESERVE_REMOVALS and re-planned
+    one-shot calls, each list against the oracle; then a list that breaks the
+    contract (a duplicate tail removal) is reported and the next batch on the
+    context is exact (the bucket counts start from zero again)."""
+    rng = np.random.default_rng(2525)
+    specs = _random_specs(rng, 60, 60000)
+    specs += [(1 << 22, 1 << 21, 4, "uniform"), ((1 << 21) + 7, 900_000, 16, "clustered"),
+              (1 << 21, 600_000, 8, "tail_heavy"), (1 << 20, 1 << 19, 4, "uniform"), (1 << 20, 600_000, 16, "uniform"),
+              ((1 << 22) + 3, (1 << 22) - 1, 4, "uniform"), (3_000_000, 140_000, 8, "tail_heavy")]
+    L = Lists(specs, seed=2600)
+    for cfg in (rz.TileConfig(512, 8, 0, 5, 1, 1), rz.TileConfig(256, 8, 0, 5, 2, 1), rz.TileConfig(512, 4, 0, 5, 2, 1)):
+        b = ctx.batch(L.pairs(), config=cfg)
+        for rep in range(2):
+            announce("batch bucketed", cfg.as_tuple(), rep)
+            L.restore()
+            b.launch()
+            L.check(oracle, f"bucketed {cfg.as_tuple()} launch {rep}")
+        b.close()
+    cfg = rz.TileConfig(512, 8, 0, 5, 1, 1)
+    b = ctx.batch(L.pairs(), config=cfg, flags=rz.RZ_PRESERVE_REMOVALS)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        L.restore()
+        torch.cuda.synchronize()
+        b.capture(stream=stream)
+        for rep in range(3):
+            announce("batch bucketed replay", rep)
+            L.restore()
+            torch.cuda.synchronize()
+            b.launch(stream=stream)
+            stream.synchronize()
+            L.check(oracle, f"bucketed replay {rep}")
+            for d, h in zip(L.dev_R, L.host_R):
+                assert torch.equal(d.cpu(), h)
+    b.close()
+    old = ctx.batch_config()
+    ctx.set_batch_config(cfg)
+    try:
+        for seed in (2700, 2800):
+            rng = np.random.default_rng(seed)
+            specs2 = _random_specs(rng, 20, 40000) + [(1 << 21, int(rng.integers(1 << 19, 1 << 20)), 4, "uniform")]
+            L2 = Lists(specs2, seed=seed)
+            announce("batch bucketed one shot", seed)
+            ctx.remove_batched(L2.pairs())
+            L2.check(oracle, f"bucketed one-shot {seed}")
+        assert ctx.check_errors() == 0
+        # a duplicate tail removal in a bucketed list
+        n, k = 1 << 21, 600_000
+        A = W.unique_u32_payload(n, seed=2900).cuda()
+        head = W.uniform_removals(n - k, k - 2, seed=2901)
+        R = torch.cat([torch.tensor([n - 1, n - 1], dtype=torch.int32), head]).cuda()
+        announce("batch bucketed mismatch")
+        ctx.remove_batched([(A, R)])
+        with pytest.raises(rz.RedzoneError) as e:
+            ctx.check_errors()
+        assert e.value.error_flags & rz.RZ_ERR_ORPHAN_MISMATCH
+        L.restore()
+        ctx.remove_batched(L.pairs())
+        L.check(oracle, "bucketed after mismatch")
+    finally:
+        ctx.set_batch_config(old)
+    assert ctx.check_errors() == 0
