@@ -213,8 +213,10 @@ def _side_stream(ctx):
 def run_batched_learned(ctx, disp, lists, specs=None, batch_max_k=BATCH_MAX_K, batch_partition=False):
     """Lists below batch_max_k removals in one persistent launch of Redzone
     items; the others launched on their own with the method the MLP picks.
-    The network's features and forward pass run on a second stream, for the
-    large lists beside the batch, and only the choice comes back to the host.
+    The batch is planned and launched first; the network's features and
+    forward pass for the large lists then run on a second stream beside it
+    (waiting only for the lists, not for the batch), and only the choice
+    comes back to the host.
     With batch_partition the network also decides for every batched list
     before the batch is planned, and a batched list it sends to the partition
     runs the batch's partition items (the stable O(n) compaction).  That is
@@ -242,12 +244,17 @@ def run_batched_learned(ctx, disp, lists, specs=None, batch_max_k=BATCH_MAX_K, b
     t0 = time.perf_counter()
     side = _side_stream(ctx)
     decide_small = batch_partition and disp.trained and len(small) > 0
-    C = None
-    if (len(large) or decide_small) and disp.trained:
-        side.wait_stream(torch.cuda.current_stream(ctx.device))  # the lists are ready on the launch stream
-        which = descs if decide_small else descs.subset(large)
+    ready = torch.cuda.Event()
+    ready.record(torch.cuda.current_stream(ctx.device))  # the lists are ready on the launch stream
+
+    def decide(which):
+        side.wait_event(ready)
         with torch.cuda.stream(side):
-            _, C = disp.predict(disp.features(which, stream=side), stream=side)
+            return disp.predict(disp.features(which, stream=side), stream=side)[1]
+
+    C = None
+    if decide_small:  # every list's method before the batch is planned
+        C = decide(descs)
     if len(small):
         sub = descs.subset(small)
         if decide_small:
@@ -255,7 +262,9 @@ def run_batched_learned(ctx, disp, lists, specs=None, batch_max_k=BATCH_MAX_K, b
                 C = C.cpu().numpy()
             sub.rows["method"] = np.where(C[small] == 3, 3, 0)
             choice[small] = sub.rows["method"]
-        ctx.remove_batched(sub)
+        ctx.remove_batched(sub)  # planned and launched first, so the device starts at once
+    if len(large) and disp.trained and C is None:
+        C = decide(descs.subset(large))  # its host-side launch overlaps the batch on the device
     if len(large):
         if C is not None:
             if not isinstance(C, np.ndarray):
