@@ -55,7 +55,7 @@ rz_status reset_scratch(rz_ctx* ctx) {
   DeviceBuffer* bufs[] = {&ctx->tail_bits, &ctx->list_bits,  &ctx->counts,    &ctx->orphans,    &ctx->status,
                           &ctx->queue,     &ctx->mstatus,    &ctx->slots,     &ctx->dense_cnt,  &ctx->dense_small,
                           &ctx->dense_ent, &ctx->dense_fine, &ctx->dense_src, &ctx->gstatus,    &ctx->ploaded,
-                          &ctx->pgcnt,     &ctx->pgdone};
+                          &ctx->pgcnt,     &ctx->pgdone,     &ctx->pcur};
   for (DeviceBuffer* b : bufs)
     if (b->ptr) RZ_HIP(hipMemset(b->ptr, 0, b->bytes));
   // every state word (epochs, counters, ready words, the error word); the
@@ -182,7 +182,7 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
                           &ctx->status,    &ctx->ctrl,      &ctx->queue,  &ctx->mstatus, &ctx->slots,
                           &ctx->dense_cnt, &ctx->dense_small, &ctx->dense_ent, &ctx->dense_fine,
                           &ctx->dense_src, &ctx->dispatch, &ctx->gstatus, &ctx->ploaded, &ctx->pgcnt,
-                          &ctx->pgdone};
+                          &ctx->pgdone, &ctx->pent, &ctx->pcur};
   for (DeviceBuffer* b : bufs)
     if (b->ptr) (void)hipFree(b->ptr);
   if (ctx->ctrl_host) (void)hipHostFree(ctx->ctrl_host);

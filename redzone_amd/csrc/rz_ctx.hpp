@@ -92,6 +92,8 @@ struct rz_ctx {
   rz::DeviceBuffer ploaded;    // partition: per tile, the epoch once its items are loaded
   rz::DeviceBuffer pgcnt;      // partition: per 64 tiles, the number of them loaded (zero between calls)
   rz::DeviceBuffer pgdone;     // partition: per 64 tiles, the epoch once all of them are loaded
+  rz::DeviceBuffer pent;       // partition's bucketed marks: per 2^19-position bin, its removals
+  rz::DeviceBuffer pcur;       // partition's bucketed marks: per bin its count, then per bin a done counter (zero between calls)
   rz::DeviceBuffer ctrl;       // control block (rz::CtrlWord)
   rz::DeviceBuffer queue;      // ticket / exit / done words of the single-list queue launch
   rz::DeviceBuffer mstatus;    // fused form: two halves of kMaxSlices look-back words, used alternately
@@ -159,10 +161,6 @@ rz_status dense_moves(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, int64_t k,
 bool dense_bucket_marks(const rz_tile_config& cfg, int64_t k, uint32_t flags);
 // dense form: whether (cfg, n, k) can be launched (checked before any kernel of a call runs)
 bool dense_shape_ok(const rz_ctx* ctx, const rz_tile_config& cfg, uint32_t esz, int64_t n, int64_t k, uint32_t flags);
-// bucketed (LDS) marks of the window [base, base + m) into an m-bit bitmap (rz_dense.hip)
-bool bucket_marks_fit(int64_t m);
-rz_status bucket_marks(rz_ctx* ctx, const int32_t* R, int64_t k, uint32_t base, int64_t m, uint32_t* bits,
-                       bool zero_bits, hipStream_t s);
 // After a failed device check: zero the scratch the kernels keep zero between
 // calls (context buffers, state words of the control block, batches), so that
 // the next call starts clean.  Synchronises the device.

@@ -664,68 +664,6 @@ static rz_status dense_launch(rz_ctx* ctx, T* A, const int32_t* R, uint32_t k, u
   return RZ_OK;
 }
 
-// Bucketed marks of the window [base, base + m): the k removals are grouped by
-// bin of 2^kBucketBinLog positions (count, scan, staged entries), then one
-// workgroup per bin sets its bits in LDS and writes its words whole.  Every
-// word of the m-bit bitmap is written (zero where nothing is removed), so
-// the bitmap need not be clear beforehand.  With zero_bits (the bitmap is
-// zero on entry) a bin's marks are shared by several workgroups that OR
-// their words in, so that about two workgroups per CU run.  Removals outside
-// the window are ignored.  Needs m <= kCoarseMax * 2^kBucketBinLog.
-bool bucket_marks_fit(int64_t m) { return m > 0 && m <= ((int64_t)kCoarseMax << kBucketBinLog); }
-
-rz_status bucket_marks(rz_ctx* ctx, const int32_t* R, int64_t k, uint32_t base, int64_t m, uint32_t* bits,
-                       bool zero_bits, hipStream_t s) {
-  // (2^20-position bins and 16 removals per thread per step write longer
-  // runs, but their LDS halves the workgroups per CU: 30-60% slower)
-  constexpr int WG = 1024, IPT = 8;
-  constexpr uint32_t S = WG * IPT;
-  if (!bucket_marks_fit(m) || k <= 0) return RZ_EINVAL;
-  rz_tile_config cfg{WG, IPT, 1024, 5, 0, 2};
-  uint32_t L, nt;
-  slice_shape(k, cfg, kMaxSlices, &L, &nt);
-  uint32_t NBt = (uint32_t)((m + ((int64_t)1 << kBucketBinLog) - 1) >> kBucketBinLog);
-  NBt = (NBt + 31u) & ~31u;
-  rz_status st;
-  if ((st = ensure(ctx, ctx->dense_cnt, (size_t)nt * NBt * 4 * 2, s)) != RZ_OK) return st;
-  if ((st = ensure(ctx, ctx->dense_small, kSmallBytes, s)) != RZ_OK) return st;
-  if ((st = ensure(ctx, ctx->dense_ent, (size_t)k * 4, s)) != RZ_OK) return st;
-  const size_t lds_tail = (size_t)S * (4 + 2);
-  const size_t lds_mark = (size_t)4 << (kBucketBinLog - 5);
-  if ((size_t)ctx->lds_max < lds_mark || (size_t)ctx->lds_max < lds_tail + 16 * 1024) return RZ_EINVAL;
-  static std::atomic<uint64_t> attr_mask{0};
-  if (attr_needed(attr_mask, ctx->device)) {
-    RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_tail_entries<WG, IPT>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_tail));
-    RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_tail_mark, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)kTailMarkLdsMax));
-    attr_done(attr_mask, ctx->device);
-  }
-  uint32_t* cnt = (uint32_t*)ctx->dense_cnt.ptr;
-  uint32_t* toff = cnt + (size_t)nt * NBt;
-  uint32_t* tbtot = (uint32_t*)ctx->dense_small.ptr + 2 * kSmallStride;
-  uint32_t* tbinbase = tbtot + kSmallStride;
-  uint32_t* ent = (uint32_t*)ctx->dense_ent.ptr;
-  hipLaunchKernelGGL((rz_dense_tail_count<WG, IPT>), dim3(nt), dim3(WG), 0, s, R, (uint32_t)k, base, (uint32_t)m, L,
-                     NBt, kBucketBinLog, cnt);
-  hipLaunchKernelGGL(rz_dense_colscan, dim3(NBt / 32), dim3(1024), 0, s, cnt, nt, NBt, toff, tbtot);
-  hipLaunchKernelGGL(rz_dense_topscan, dim3(1), dim3(1024), 0, s, tbtot, NBt, tbinbase, (const uint64_t*)nullptr, 0u,
-                     (uint64_t*)nullptr, (uint32_t*)nullptr, 0);
-  hipLaunchKernelGGL((rz_dense_tail_entries<WG, IPT>), dim3(nt), dim3(WG), lds_tail, s, R, (uint32_t)k, base,
-                     (uint32_t)m, L, NBt, kBucketBinLog, toff, tbinbase, ent);
-  uint32_t G = 1;
-  if (zero_bits) {
-    const uint32_t want = 2u * (uint32_t)ctx->cu_count;
-    G = (want + NBt - 1) / NBt;
-    if (G > 16) G = 16;
-    if (G < 1) G = 1;
-  }
-  hipLaunchKernelGGL(rz_dense_tail_mark, dim3(NBt * G), dim3(1024), lds_mark, s, ent, tbinbase, (uint32_t)m, bits,
-                     (uint64_t*)nullptr, NBt, G, kBucketBinLog);
-  RZ_HIP(hipGetLastError());
-  return RZ_OK;
-}
-
 bool dense_bucket_marks(const rz_tile_config& cfg, int64_t k, uint32_t flags) {
   return cfg.variant == 5 && cfg.chunk == 2 && k <= ((int64_t)kCoarseMax << kTailBinLog) && !(flags & RZ_SORTED);
 }

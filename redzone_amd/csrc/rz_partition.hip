@@ -29,6 +29,8 @@
 //                 every tile overlapping its output range has loaded.  It
 //                 stages the kept items in LDS and writes them as whole
 //                 vectors.
+#include <atomic>
+
 #include "rz_compact_dev.hpp"
 #include "rz_ctx.hpp"
 #include "rz_device.hpp"
@@ -365,6 +367,121 @@ rz_status launch_validate(rz_ctx* ctx, const int32_t* R, int64_t n, int64_t k, u
 // launch
 // ---------------------------------------------------------------------------
 constexpr int kPartWG = 256, kPartIPT = 16;
+// ---------------------------------------------------------------------------
+// Bucketed marks of the partition, two kernels, over bins of 2^blog
+// positions (2^15 to 2^19, so that a list has 256 to 1,024 bins).
+// rz_part_append: a workgroup takes 1,024 x IPT removals, counts them per bin in
+// LDS, reserves room in each bin's entry list (2^blog entries, a bin's
+// positions) with one atomic per bin, sorts them by bin in LDS and writes them
+// out bin by bin, so consecutive threads write consecutive words.
+// rz_part_bits: G workgroups per bin (G = 1 from 256 bins up) set the bin's
+// bits in an LDS bitmap and store its words into the list bitmap (zero on
+// entry; OR-ed when G > 1); the last of a bin's workgroups puts the bin's
+// count back to zero, and workgroup 0 starts the compaction's epoch.  No scans
+// and no epoch launch: two launches where the count / scan / entries / bits
+// pipeline took five and the epoch a sixth.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kPartMinBinLog = 15, kPartMaxBinLog = 19;  // 4 to 64 KiB LDS bitmaps
+constexpr uint32_t kPartMaxBins = 1024;                       // lists of up to 2^29 items
+constexpr uint32_t kAppWG = 1024;
+// removals per thread: the bigger the stage (dynamic LDS, 4 bytes a removal),
+// the longer each bin's runs of consecutive words and the fewer count atomics
+constexpr int kAppIPT = 16, kAppIPTSmall = 8;
+
+template <int IPT>
+__global__ __launch_bounds__(1024) void rz_part_append(const int32_t* __restrict__ R, uint32_t k, uint32_t n,
+                                                        uint32_t blog, uint32_t nbins, uint32_t* __restrict__ ent,
+                                                        uint32_t* __restrict__ cur) {
+  constexpr uint32_t kAppS = kAppWG * IPT;
+  __shared__ uint32_t s_cnt[kPartMaxBins], s_lbase[kPartMaxBins], s_gbase[kPartMaxBins];
+  __shared__ uint32_t s_tmp[kAppWG / kWave + 1];
+  extern __shared__ __align__(16) unsigned char dyn[];
+  uint32_t* s_stage = (uint32_t*)dyn;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t a = blockIdx.x * kAppS, b = min(a + kAppS, k);
+  for (uint32_t w = tid; w < nbins; w += kAppWG) s_cnt[w] = 0u;
+  __syncthreads();
+  uint32_t r[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t i = a + j * kAppWG + tid;
+    r[j] = i < b ? (uint32_t)R[i] : 0xffffffffu;
+  }
+#pragma unroll
+  for (int j = 0; j < IPT; ++j)
+    if (r[j] < n) atomicAdd(&s_cnt[r[j] >> blog], 1u);
+  __syncthreads();
+  // exclusive prefix over the bins (one bin per thread), then room in each bin's list
+  uint32_t total;
+  const uint32_t c = tid < nbins ? s_cnt[tid] : 0u;
+  const uint32_t ex = block_excl_scan<kAppWG>(c, s_tmp, &total);
+  if (tid < nbins) {
+    s_lbase[tid] = ex;
+    s_gbase[tid] = c ? add_relaxed(&cur[tid], c) : 0u;
+    s_cnt[tid] = 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < IPT; ++j)
+    if (r[j] < n) {
+      const uint32_t bz = r[j] >> blog;
+      s_stage[s_lbase[bz] + atomicAdd(&s_cnt[bz], 1u)] = r[j];
+    }
+  __syncthreads();
+  for (uint32_t t = tid; t < total; t += kAppWG) {
+    const uint32_t v = s_stage[t];
+    const uint32_t bz = v >> blog;
+    const uint32_t g = s_gbase[bz] + (t - s_lbase[bz]);
+    if (g < (1u << blog)) ent[((size_t)bz << blog) + g] = v;  // more only with duplicate removals
+  }
+}
+
+__global__ __launch_bounds__(1024) void rz_part_bits(const uint32_t* __restrict__ ent, uint32_t* __restrict__ cur,
+                                                      uint32_t* __restrict__ bdone, uint32_t n, uint32_t blog,
+                                                      uint32_t G, uint32_t* __restrict__ bits,
+                                                      uint32_t* __restrict__ ctrl) {
+  const uint32_t W = 1u << (blog - 5), cap = 1u << blog;
+  extern __shared__ __align__(16) unsigned char dyn[];
+  uint32_t* lb = (uint32_t*)dyn;
+  const uint32_t bin = blockIdx.x / G, g = blockIdx.x % G, tid = threadIdx.x;
+  for (uint32_t w = tid; w < W; w += 1024u) lb[w] = 0u;
+  const uint32_t c = min(cur[bin], cap);
+  __syncthreads();
+  const uint32_t share = (c + G - 1) / G;
+  const uint32_t s = min(g * share, c), e = min(s + share, c);
+  const uint32_t* be = ent + ((size_t)bin << blog);
+  const uint32_t z0 = bin << blog;
+  constexpr int U = 8;  // loads in flight per thread, then their LDS atomics
+  for (uint32_t q0 = s + tid; q0 < e; q0 += 1024u * U) {
+    uint32_t r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t q = q0 + u * 1024u;
+      r[u] = q < e ? be[q] - z0 : 0xffffffffu;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (r[u] < cap) atomicOr(&lb[r[u] >> 5], 1u << (r[u] & 31));
+  }
+  __syncthreads();
+  const uint32_t nw = (n + 31u) >> 5;
+  for (uint32_t w = tid; w < W; w += 1024u) {
+    const uint32_t gw = (z0 >> 5) + w;
+    if (gw < nw && lb[w]) {
+      if (G == 1) bits[gw] = lb[w];
+      else or_relaxed(&bits[gw], lb[w]);
+    }
+  }
+  if (tid == 0) {
+    // every workgroup of the bin has read its count: the last puts it back to zero
+    if (add_relaxed(&bdone[bin], 1u) == G - 1u) {
+      st_relaxed(&cur[bin], 0u);
+      st_relaxed(&bdone[bin], 0u);
+    }
+    if (blockIdx.x == 0) add_relaxed(&ctrl[kCtrlEpoch], 1u);
+  }
+}
+
 // Marks of lists of at least kPartBucketMinN items and kPartBucketMinK removals
 // are grouped by bitmap region first: there, one random atomic per removal
 // costs a DRAM read-modify-write (the bitmap is beyond the caches).  Measured
@@ -402,12 +519,41 @@ static rz_status partition_impl(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, 
   const int64_t gw = (k + 255) / 256;
   const dim3 g1((uint32_t)(gw < (int64_t)ctx->cu_count * 8 ? (gw > 0 ? gw : 1) : ctx->cu_count * 8));
   if ((st = prof_mark(ctx, flags, 0, s)) != RZ_OK) return st;
-  const bool bucketed = k >= kPartBucketMinK && n >= kPartBucketMinN && bucket_marks_fit(n);
+  const bool bucketed = k >= kPartBucketMinK && n >= kPartBucketMinN && n <= ((int64_t)kPartMaxBins << kPartMaxBinLog);
   if (bucketed) {
-    // every bitmap word is written; the epoch bump comes from an empty mark launch
     // the bitmap is zero here: every call leaves the words it used clear
-    if ((st = bucket_marks(ctx, R, k, 0u, n, bits, true, s)) != RZ_OK) return st;
-    hipLaunchKernelGGL(rz_mark_all, dim3(1), dim3(256), 0, s, R, 0u, (uint32_t)n, bits, ctrl);
+    uint32_t blog = kPartMinBinLog;
+    while (blog < kPartMaxBinLog && (n >> blog) > 512) ++blog;
+    const uint32_t nbins = (uint32_t)((n + (1ll << blog) - 1) >> blog);
+    if ((st = ensure(ctx, ctx->pent, (size_t)nbins << (blog + 2), s)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->pcur, (size_t)nbins * 8, s)) != RZ_OK) return st;  // counts, then done counters
+    constexpr size_t lds_max = (size_t)4 << (kPartMaxBinLog - 5);
+    const size_t lds_bits = (size_t)4 << (blog - 5);
+    // the big stage where the device gives a workgroup its LDS (plus about 12 KiB static)
+    constexpr size_t lds_app = (size_t)4 * kAppWG * kAppIPT, lds_app_small = (size_t)4 * kAppWG * kAppIPTSmall;
+    const bool big = lds_app + 16384 <= (size_t)ctx->lds_max;
+    static std::atomic<uint64_t> attr_mask{0};
+    const int dev = ctx->device;
+    if (dev < 0 || dev >= 64 || !(attr_mask.load(std::memory_order_relaxed) & (1ull << dev))) {
+      RZ_HIP(hipFuncSetAttribute((const void*)rz_part_bits, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
+      if (big)
+        RZ_HIP(hipFuncSetAttribute((const void*)rz_part_append<kAppIPT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds_app));
+      if (dev >= 0 && dev < 64) attr_mask.fetch_or(1ull << dev, std::memory_order_relaxed);
+    }
+    uint32_t* cur = (uint32_t*)ctx->pcur.ptr;
+    if (big)
+      hipLaunchKernelGGL(rz_part_append<kAppIPT>, dim3((uint32_t)((k + kAppWG * kAppIPT - 1) / (kAppWG * kAppIPT))),
+                         dim3(kAppWG), lds_app, s, R, (uint32_t)k, (uint32_t)n, blog, nbins, (uint32_t*)ctx->pent.ptr,
+                         cur);
+    else
+      hipLaunchKernelGGL(rz_part_append<kAppIPTSmall>,
+                         dim3((uint32_t)((k + kAppWG * kAppIPTSmall - 1) / (kAppWG * kAppIPTSmall))), dim3(kAppWG),
+                         lds_app_small, s, R, (uint32_t)k, (uint32_t)n, blog, nbins, (uint32_t*)ctx->pent.ptr, cur);
+    uint32_t G = ((uint32_t)ctx->cu_count + nbins - 1) / nbins;  // about one workgroup per CU
+    G = G < 1 ? 1 : (G > 16 ? 16 : G);
+    hipLaunchKernelGGL(rz_part_bits, dim3(nbins * G), dim3(1024), lds_bits, s, (const uint32_t*)ctx->pent.ptr, cur,
+                       cur + nbins, (uint32_t)n, blog, G, bits, ctrl);
   } else {
     hipLaunchKernelGGL(rz_mark_all, g1, dim3(256), 0, s, R, (uint32_t)k, (uint32_t)n, bits, ctrl);
   }
