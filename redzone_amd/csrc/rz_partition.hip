@@ -407,9 +407,9 @@ __global__ __launch_bounds__(1024) void rz_part_append(const int32_t* __restrict
     const uint32_t i = a + j * kAppWG + tid;
     r[j] = i < b ? (uint32_t)R[i] : 0xffffffffu;
   }
+  // a wave whose removals share one bin (a run) counts them with one atomic
 #pragma unroll
-  for (int j = 0; j < IPT; ++j)
-    if (r[j] < n) atomicAdd(&s_cnt[r[j] >> blog], 1u);
+  for (int j = 0; j < IPT; ++j) lds_bin_count(s_cnt, r[j] >> blog, r[j] < n);
   __syncthreads();
   // exclusive prefix over the bins (one bin per thread), then room in each bin's list
   uint32_t total;
@@ -422,11 +422,11 @@ __global__ __launch_bounds__(1024) void rz_part_append(const int32_t* __restrict
   }
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < IPT; ++j)
-    if (r[j] < n) {
-      const uint32_t bz = r[j] >> blog;
-      s_stage[s_lbase[bz] + atomicAdd(&s_cnt[bz], 1u)] = r[j];
-    }
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t bz = r[j] >> blog;
+    const uint32_t rank = lds_bin_add(s_cnt, bz, r[j] < n);
+    if (r[j] < n) s_stage[s_lbase[bz] + rank] = r[j];
+  }
   __syncthreads();
   for (uint32_t t = tid; t < total; t += kAppWG) {
     const uint32_t v = s_stage[t];
@@ -447,17 +447,30 @@ __global__ __launch_bounds__(1024) void rz_part_bits(const uint32_t* __restrict_
   for (uint32_t w = tid; w < W; w += 1024u) lb[w] = 0u;
   const uint32_t c = min(cur[bin], cap);
   __syncthreads();
-  const uint32_t share = (c + G - 1) / G;
+  const uint32_t share = ((c + G - 1) / G + 3u) & ~3u;  // a multiple of 4: 16-byte vector loads
   const uint32_t s = min(g * share, c), e = min(s + share, c);
-  const uint32_t* be = ent + ((size_t)bin << blog);
+  const uint32_t* be = ent + ((size_t)bin << blog);  // 16-byte aligned: bins start at multiples of 2^15
   const uint32_t z0 = bin << blog;
-  constexpr int U = 8;  // loads in flight per thread, then their LDS atomics
-  for (uint32_t q0 = s + tid; q0 < e; q0 += 1024u * U) {
+  // Each thread takes U consecutive entries.  A run of consecutive positions
+  // (clustered removals) then puts the lanes of one LDS atomic U positions
+  // apart, so at most two lanes share a bitmap word, where consecutive entries
+  // per lane would put 32 lanes on one word.
+  constexpr int U = 16;
+  for (uint32_t q0 = s + tid * U; q0 < e; q0 += 1024u * U) {
     uint32_t r[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t q = q0 + u * 1024u;
-      r[u] = q < e ? be[q] - z0 : 0xffffffffu;
+    for (int v = 0; v < U / 4; ++v) {
+      const uint32_t q = q0 + 4u * v;
+      if (q + 4u <= e) {
+        const uint4 x = *(const uint4*)(be + q);
+        r[4 * v] = x.x - z0;
+        r[4 * v + 1] = x.y - z0;
+        r[4 * v + 2] = x.z - z0;
+        r[4 * v + 3] = x.w - z0;
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) r[4 * v + i] = q + i < e ? be[q + i] - z0 : 0xffffffffu;
+      }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
@@ -483,11 +496,12 @@ __global__ __launch_bounds__(1024) void rz_part_bits(const uint32_t* __restrict_
 }
 
 // Marks of lists of at least kPartBucketMinN items and kPartBucketMinK removals
-// are grouped by bitmap region first: there, one random atomic per removal
-// costs a DRAM read-modify-write (the bitmap is beyond the caches).  Measured
-// on 4-byte items: n=2^28, k=n/2 5.76 -> 1.64 ms; n=2^24, k=n/10
-// 131 -> 110 us; below 2^23 items the atomics stay in cache and win.
-constexpr int64_t kPartBucketMinK = 1 << 20, kPartBucketMinN = 1 << 23;
+// are grouped by bitmap region first: one random device atomic per removal
+// runs at about 40 G/s, where the two bucketing kernels stream.  Whole calls
+// on 4-byte items, random atomics -> bucketed: n=2^22, k=n/2 105 -> 43 us
+// (clustered 160 -> 47 us); n=6,000,000, k=0.8n 202 -> 55 us; n=2^21,
+// k=n/4 40 -> 36 us.  At n=2^20, k=2^18 the atomics still win (30 vs 41 us).
+constexpr int64_t kPartBucketMinK = 1 << 19, kPartBucketMinN = 1 << 20;
 
 // 1024 threads x 4 vectors: 64 KiB tiles (staged in LDS); bigger tiles take
 // fewer look-back hops per byte (a 1 GiB probe: 16 KiB tiles 0.70 ms,

@@ -91,11 +91,14 @@ def test_partition_full_size(ctx, oracle):
 
 @pytest.mark.parametrize("esz", [4, 8, 16])
 def test_partition_bucketed_marks(ctx, oracle, esz):
-    """Lists of at least 2^23 items with at least 2^20 removals group their
-    marks by bitmap region (one LDS bitmap per 2^19 positions, every bitmap
-    word written whole); list lengths that end inside a region, clustered
-    removals that fill whole regions, and k = n - 1."""
+    """Lists of at least 2^20 items with at least 2^19 removals group their
+    marks by bitmap region (bins of 2^15 to 2^19 positions, 256 to 1,024 of
+    them from 2^23 items up, several workgroups per bin below); list lengths
+    that end inside a region, clustered removals that fill whole regions, the
+    bin count changing from call to call, and k = n - 1."""
     for n, k, pattern in [((1 << 23) + 12345, 1 << 20, "uniform"), ((1 << 23) + 12345, 3 << 21, "clustered"),
+                          (1 << 20, 1 << 19, "clustered"), ((1 << 21) + 777, 1_500_000, "uniform"),
+                          (6_000_001, 4_800_000, "clustered"), ((1 << 22) + 3, 1 << 19, "tail_heavy"),
                           (1 << 23, (1 << 23) - 1, "uniform"), ((3 << 22) + 1, 1 << 22, "tail_heavy")]:
         announce("partition bucketed", esz, n, k, pattern)
         _run(ctx, oracle, W.payload(esz, n, seed=40), W.removals(pattern, n, k, seed=41))
