@@ -382,6 +382,23 @@ rz_status rz_dispatch_set_params(rz_ctx* ctx, const float* h_params);
  * (README.md:6; paper p.14: Redzone below k = n/2, O(n) above). */
 rz_status rz_remove_auto(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, int32_t* d_removals, int64_t k,
                          uint32_t flags, int32_t* method, void* stream);
+/* The network's method for many lists, issued without waiting (no counterpart
+ * in the reference, whose fixed rule is README.md:6).  rz_dispatch_choose
+ * copies the host descriptors h_lists (method fields ignored) to the device
+ * through pinned staging, computes their features and the forward pass on the
+ * stream (in stream order after the removal lists' producers), copies the
+ * choices back into pinned staging and records an event, then returns.
+ * d_params: RZ_MLP_PARAMS device floats, or null for the parameters installed
+ * by rz_dispatch_set_params; with neither, the k < n/2 rule applies.
+ * rz_dispatch_collect waits for that event only and writes one RZ_METHOD_* per
+ * list into h_choice (nlists must match the last rz_dispatch_choose); lists
+ * with k = 0 or k = n get RZ_METHOD_REDZONE, a value out of range the rule.
+ * One choice per context is outstanding at a time: a new rz_dispatch_choose
+ * waits for the previous one's copy and replaces it.  Issued ahead of other
+ * host work (a batch's plan), the choice is ready when that work is done. */
+rz_status rz_dispatch_choose(rz_ctx* ctx, const float* d_params, const rz_list_desc* h_lists, int32_t nlists,
+                             void* stream);
+rz_status rz_dispatch_collect(rz_ctx* ctx, int32_t* h_choice, int32_t nlists);
 
 /* ---- allocation-free removal -------------------------------------------- */
 /* No counterpart in the reference.  For callers that cannot allocate device

@@ -119,17 +119,6 @@ static rz_status check_args(rz_ctx* ctx, const void* data, int64_t n, uint32_t e
   return RZ_OK;
 }
 
-struct DeviceGuard {
-  int prev = -1;
-  explicit DeviceGuard(int dev) {
-    if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
-    else prev = -1;
-  }
-  ~DeviceGuard() {
-    if (prev >= 0) (void)hipSetDevice(prev);
-  }
-};
-
 }  // namespace rz
 
 using namespace rz;
@@ -182,11 +171,14 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
                           &ctx->status,    &ctx->ctrl,      &ctx->queue,  &ctx->mstatus, &ctx->slots,
                           &ctx->dense_cnt, &ctx->dense_small, &ctx->dense_ent, &ctx->dense_fine,
                           &ctx->dense_src, &ctx->dispatch, &ctx->gstatus, &ctx->ploaded, &ctx->pgcnt,
-                          &ctx->pgdone, &ctx->pent, &ctx->pcur};
+                          &ctx->pgdone, &ctx->pent, &ctx->pcur, &ctx->choose};
+  if (ctx->choose_ev) (void)hipEventSynchronize(ctx->choose_ev);  // its copy back has left the staging
   for (DeviceBuffer* b : bufs)
     if (b->ptr) (void)hipFree(b->ptr);
   if (ctx->ctrl_host) (void)hipHostFree(ctx->ctrl_host);
   if (ctx->auto_host) (void)hipHostFree(ctx->auto_host);
+  if (ctx->choose_host) (void)hipHostFree(ctx->choose_host);
+  if (ctx->choose_ev) (void)hipEventDestroy(ctx->choose_ev);
   for (hipEvent_t e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   delete ctx;

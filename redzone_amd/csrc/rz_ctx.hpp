@@ -107,6 +107,12 @@ struct rz_ctx {
   rz::DeviceBuffer dispatch;   // network parameters, one list descriptor, its features and the choice
   bool dispatch_ready = false; // parameters installed (else the k < n/2 rule)
   uint8_t* auto_host = nullptr;  // pinned staging of the descriptor and the choice
+  rz::DeviceBuffer choose;       // rz_dispatch_choose: descriptors, features, choices
+  uint8_t* choose_host = nullptr;  // its pinned staging: descriptors, then choices
+  size_t choose_host_bytes = 0;
+  hipEvent_t choose_ev = nullptr;  // recorded after the choices' copy back
+  int32_t choose_n = -1;           // lists of the choice not yet collected (-1: none)
+  bool choose_dev = false;         // that choice is the network's (else the host's k < n/2 rule)
   rz_tile_config batch_cfg;    // default launch shape of batches
   rz_batch* last_batch = nullptr;  // plan of the last rz_remove_batched call
   std::vector<rz_batch*> batches;  // batches made by rz_batch_create (their scratch is reset with the context's)
@@ -123,6 +129,17 @@ struct rz_ctx {
 namespace rz {
 
 rz_status ensure(rz_ctx* ctx, DeviceBuffer& b, size_t bytes, hipStream_t s);
+// Makes `dev` the current device for a scope and restores the previous one.
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
+    else prev = -1;
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
 inline int esz_index(uint32_t esz) { return esz == 4 ? 0 : esz == 8 ? 1 : esz == 16 ? 2 : -1; }
 inline int log2_floor(int64_t v) {
   int r = 0;

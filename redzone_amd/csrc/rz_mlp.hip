@@ -8,6 +8,7 @@
 // for each layer, in order.  Training minimises the masked mean squared error
 // on measured log times with Adam.  Per-sample gradients are written out and
 // summed in a fixed order, so a gradient is the same bits on every run.
+#include <cstring>
 #include <mutex>
 
 #include "rz_ctx.hpp"
@@ -345,6 +346,84 @@ static rz_status auto_choose(rz_ctx* ctx, void* d_data, int64_t n, uint32_t esz,
   }
   if (prev >= 0 && prev != ctx->device) (void)hipSetDevice(prev);
   return st;
+}
+
+static int32_t rule_method(const rz_list_desc& d) {
+  return 2 * d.k < d.n ? RZ_METHOD_REDZONE : RZ_METHOD_PARTITION;
+}
+
+static size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+rz_status rz_dispatch_choose(rz_ctx* ctx, const float* d_params, const rz_list_desc* h_lists, int32_t nlists,
+                             void* stream) {
+  if (!ctx || nlists < 0 || (nlists > 0 && !h_lists)) return RZ_EINVAL;
+  for (int32_t i = 0; i < nlists; ++i) {
+    const rz_list_desc& d = h_lists[i];
+    if (d.n < 0 || d.k < 0 || d.k > d.n || esz_index(d.elem_bytes) < 0 || (d.k > 0 && !d.removals))
+      return RZ_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  DeviceGuard g(ctx->device);
+  hipStream_t s = (hipStream_t)stream;
+  // an uncollected choice still copying back must leave the staging area first
+  if (ctx->choose_n >= 0 && ctx->choose_dev) RZ_HIP(hipEventSynchronize(ctx->choose_ev));
+  ctx->choose_n = -1;
+  const size_t dbytes = round256((size_t)nlists * sizeof(rz_list_desc));
+  const size_t fbytes = round256((size_t)nlists * RZ_MLP_FEATURES * 4);
+  const size_t cbytes = round256((size_t)nlists * 4 + 4);
+  if (dbytes + cbytes > ctx->choose_host_bytes) {
+    if (ctx->choose_host) RZ_HIP(hipHostFree(ctx->choose_host));
+    ctx->choose_host = nullptr;
+    ctx->choose_host_bytes = 0;
+    size_t cap = 4096;
+    while (cap < dbytes + cbytes) cap *= 2;
+    if (hipHostMalloc((void**)&ctx->choose_host, cap, hipHostMallocDefault) != hipSuccess) {
+      ctx->choose_host = nullptr;
+      return RZ_ENOMEM;
+    }
+    ctx->choose_host_bytes = cap;
+  }
+  if (nlists) memcpy(ctx->choose_host, h_lists, (size_t)nlists * sizeof(rz_list_desc));
+  const float* params = d_params ? d_params : (ctx->dispatch_ready ? (const float*)ctx->dispatch.ptr : nullptr);
+  if (!params || nlists == 0) {  // no network: the rule, on the host, at collection
+    ctx->choose_dev = false;
+    ctx->choose_n = nlists;
+    return RZ_OK;
+  }
+  rz_status st;
+  if ((st = ensure(ctx, ctx->choose, dbytes + fbytes + cbytes, s)) != RZ_OK) return st;
+  char* base = (char*)ctx->choose.ptr;
+  RZ_HIP(hipMemcpyAsync(base, ctx->choose_host, (size_t)nlists * sizeof(rz_list_desc), hipMemcpyHostToDevice, s));
+  if ((st = rz_features(ctx, (const rz_list_desc*)base, nlists, (float*)(base + dbytes), s)) != RZ_OK) return st;
+  if ((st = rz_mlp_forward(ctx, params, (const float*)(base + dbytes), nlists, nullptr,
+                           (int32_t*)(base + dbytes + fbytes), s)) != RZ_OK)
+    return st;
+  RZ_HIP(hipMemcpyAsync(ctx->choose_host + dbytes, base + dbytes + fbytes, (size_t)nlists * 4, hipMemcpyDeviceToHost,
+                        s));
+  if (!ctx->choose_ev) RZ_HIP(hipEventCreateWithFlags(&ctx->choose_ev, hipEventDisableTiming));
+  RZ_HIP(hipEventRecord(ctx->choose_ev, s));
+  ctx->choose_dev = true;
+  ctx->choose_n = nlists;
+  return RZ_OK;
+}
+
+rz_status rz_dispatch_collect(rz_ctx* ctx, int32_t* h_choice, int32_t nlists) {
+  if (!ctx || nlists < 0 || (nlists > 0 && !h_choice)) return RZ_EINVAL;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (ctx->choose_n < 0 || ctx->choose_n != nlists) return RZ_EINVAL;
+  DeviceGuard g(ctx->device);
+  if (ctx->choose_dev) RZ_HIP(hipEventSynchronize(ctx->choose_ev));
+  const rz_list_desc* hd = (const rz_list_desc*)ctx->choose_host;
+  const int32_t* hc = (const int32_t*)(ctx->choose_host + round256((size_t)nlists * sizeof(rz_list_desc)));
+  for (int32_t i = 0; i < nlists; ++i) {
+    const rz_list_desc& d = hd[i];
+    int32_t m = ctx->choose_dev ? hc[i] : rule_method(d);
+    if (m < 0 || m >= RZ_MLP_OUTPUTS) m = rule_method(d);
+    if (d.k == 0 || d.k == d.n) m = RZ_METHOD_REDZONE;  // nothing to choose: no items move
+    h_choice[i] = m;
+  }
+  ctx->choose_n = -1;
+  return RZ_OK;
 }
 
 rz_status rz_remove_auto(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_bytes, int32_t* d_removals, int64_t k,

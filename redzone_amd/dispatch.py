@@ -93,6 +93,7 @@ class Dispatcher:
         self.params = torch.from_numpy(p.copy()).cuda(ctx.device)
         self.trained = params is not None
         self.train_log = []
+        self._pending = None  # the choice issued by choose_async
 
     # ---- features -------------------------------------------------------
     def features(self, lists, stream=None):
@@ -129,6 +130,30 @@ class Dispatcher:
             return np.array([rule_choice(a, b) for a, b in zip(n, k)], np.int32)
         _, C = self.predict(X)
         return C.cpu().numpy()
+
+    def choose_async(self, lists, stream=None):
+        """Issues the method choice for a ListDescs (rz_dispatch_choose, with
+        this dispatcher's parameters) on the stream without waiting; collect()
+        returns it.  An untrained dispatcher applies the k < n/2 rule."""
+        if not self.trained:
+            n, k = lists.rows["n"], lists.rows["k"]
+            rule = np.array([rule_choice(a, b) for a, b in zip(n, k)], np.int32)
+            self._pending = np.where((k == 0) | (k == n), RULE_REDZONE, rule).astype(np.int32)
+            return
+        self._pending = len(lists)
+        self.ctx._call("rz_dispatch_choose", self.ctx.handle, _ptr(self.params), lists.as_ctypes(), len(lists),
+                       _stream_ptr(stream, self.ctx.device))
+
+    def collect(self):
+        """The choice issued by choose_async (int32 RZ_METHOD_* per list)."""
+        if isinstance(self._pending, np.ndarray):
+            out, self._pending = self._pending, None
+            return out
+        out = np.empty(self._pending, np.int32)
+        self.ctx._call("rz_dispatch_collect", self.ctx.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                       len(out))
+        self._pending = None
+        return out
 
     # ---- training -------------------------------------------------------
     def grad(self, X, T, M):

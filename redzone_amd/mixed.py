@@ -201,22 +201,19 @@ def run_plain(ctx, lists, specs=None):
 BATCH_MAX_K = 1 << 22
 
 
-def _side_stream(ctx):
-    import torch
-
-    st = getattr(ctx, "_side_stream", None)
-    if st is None:
-        st = ctx._side_stream = torch.cuda.Stream(ctx.device)  # default priority
-    return st
-
-
 def run_batched_learned(ctx, disp, lists, specs=None, batch_max_k=BATCH_MAX_K, batch_partition=False):
     """Lists below batch_max_k removals in one persistent launch of Redzone
     items; the others launched on their own with the method the MLP picks.
-    The batch is planned and launched first; the network's features and
-    forward pass for the large lists then run on a second stream beside it
-    (waiting only for the lists, not for the batch), and only the choice
-    comes back to the host.
+    The network's choice for the large lists is issued first, in one library
+    call that returns at once (rz_dispatch_choose: features, forward pass and
+    the copy back of the choices, in stream order); the batch is then planned
+    and launched, and while the host plans, the device computes the choice, so
+    that collecting it waits for nothing and the large lists queue up behind
+    the batch.  (Measured on the 10,000-list stream: deciding after the
+    batch's launch left the device idle between the batch and the large
+    lists, because the persistent batch holds every CU until it ends, and
+    deciding first through separate feature and forward calls from Python
+    cost more host time than the batch's plan.)
     With batch_partition the network also decides for every batched list
     before the batch is planned, and a batched list it sends to the partition
     runs the batch's partition items (the stable O(n) compaction).  That is
@@ -225,14 +222,12 @@ def run_batched_learned(ctx, disp, lists, specs=None, batch_max_k=BATCH_MAX_K, b
     every list adds 26 ms before the batch), because a partition list marks
     all k removals and streams all n items where Redzone items mark the tail's
     removals and write the holes; hence the default.  (The large lists' own
-    launches were also tried on the second stream, beside the batch, through a
+    launches were also tried on a second stream, beside the batch, through a
     second context: no gain, the two share the memory system.)  Wall time (s)
     of everything including the decision, the descriptor upload and the batch
     plan; returns (wall, method that ran per list).  lists: [(data,
     removals), ...] or a ListDescs."""
     import torch
-
-    from .dispatch import rule_choice
 
     descs = _as_descs(ctx, lists)
     n, k = descs.rows["n"], descs.rows["k"]
@@ -242,37 +237,21 @@ def run_batched_learned(ctx, disp, lists, specs=None, batch_max_k=BATCH_MAX_K, b
     s = _stream_arg(ctx)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    side = _side_stream(ctx)
     decide_small = batch_partition and disp.trained and len(small) > 0
-    ready = torch.cuda.Event()
-    ready.record(torch.cuda.current_stream(ctx.device))  # the lists are ready on the launch stream
-
-    def decide(which):
-        side.wait_event(ready)
-        with torch.cuda.stream(side):
-            return disp.predict(disp.features(which, stream=side), stream=side)[1]
-
     C = None
     if decide_small:  # every list's method before the batch is planned
-        C = decide(descs)
+        disp.choose_async(descs)
+        C = disp.collect()
+    elif len(large):
+        disp.choose_async(descs.subset(large))  # computed on the device while the batch is planned
     if len(small):
         sub = descs.subset(small)
         if decide_small:
-            with torch.cuda.stream(side):
-                C = C.cpu().numpy()
             sub.rows["method"] = np.where(C[small] == 3, 3, 0)
             choice[small] = sub.rows["method"]
-        ctx.remove_batched(sub)  # planned and launched first, so the device starts at once
-    if len(large) and disp.trained and C is None:
-        C = decide(descs.subset(large))  # its host-side launch overlaps the batch on the device
+        ctx.remove_batched(sub)
     if len(large):
-        if C is not None:
-            if not isinstance(C, np.ndarray):
-                with torch.cuda.stream(side):
-                    C = C.cpu().numpy()
-            choice[large] = C[large] if decide_small else C
-        else:
-            choice[large] = [rule_choice(a, b) for a, b in zip(n[large], k[large])]
+        choice[large] = C[large] if decide_small else disp.collect()
         _launch_each(ctx, descs.rows[large].tolist(), [int(choice[i]) for i in large], s)
     torch.cuda.synchronize()
     return time.perf_counter() - t0, choice

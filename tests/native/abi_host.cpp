@@ -197,6 +197,20 @@ int main() {
     CHECK(rz_remove_sized(ctx, dA, 64, 4, dR, 4, nullptr, dR, 0, s) == RZ_EINVAL, "rz_remove_sized needs d_n");
     CHECK(rz_remove_sized(ctx, dA, 64, 4, dR, 4, dR, dR, RZ_VALIDATE, s) == RZ_EINVAL,
           "rz_remove_sized cannot validate (k is on the device)");
+    // the batched method choice: no network is installed in this context, so the k < n/2 rule
+    int32_t choice[4] = {-1, -1, -1, -1};
+    CHECK(rz_dispatch_collect(ctx, choice, 4) == RZ_EINVAL, "rz_dispatch_collect with nothing issued");
+    const rz_list_desc bad[1] = {{dA, dR, 3, 4, 4, 0}};
+    CHECK(rz_dispatch_choose(ctx, nullptr, bad, 1, s) == RZ_EINVAL, "rz_dispatch_choose with k > n");
+    const rz_list_desc lists[4] = {{dA, dR, 64, 4, 4, 0}, {dA, dR, 64, 40, 4, 0}, {dA, nullptr, 64, 0, 4, 0},
+                                   {dA, dR, 64, 64, 4, 0}};
+    CHECK(rz_dispatch_choose(ctx, nullptr, lists, 4, s) == RZ_OK, "rz_dispatch_choose");
+    CHECK(rz_dispatch_collect(ctx, choice, 3) == RZ_EINVAL, "rz_dispatch_collect with another list count");
+    CHECK(rz_dispatch_collect(ctx, choice, 4) == RZ_OK, "rz_dispatch_collect");
+    CHECK(choice[0] == RZ_METHOD_REDZONE && choice[1] == RZ_METHOD_PARTITION && choice[2] == RZ_METHOD_REDZONE &&
+              choice[3] == RZ_METHOD_REDZONE,
+          "the rule's choices (k = 0 and k = n: Redzone)");
+    CHECK(rz_dispatch_collect(ctx, choice, 4) == RZ_EINVAL, "a choice is collected once");
     HIPCHECK(hipFree(dA));
     HIPCHECK(hipFree(dR));
   }

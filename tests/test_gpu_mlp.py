@@ -83,3 +83,49 @@ def test_untrained_dispatcher_uses_rule(ctx):
     disp = D.Dispatcher(ctx)
     ch = disp.choose(None, n=[100, 100, 100], k=[10, 50, 90])
     assert list(ch) == [D.RULE_REDZONE, D.RULE_PARTITION, D.RULE_PARTITION]
+
+
+def _mixed_lists(seed, count):
+    rng = np.random.default_rng(seed)
+    lists = []
+    for i in range(count):
+        n = int(2 ** rng.uniform(4, 18))
+        k = [0, n][i % 2] if i % 7 == 0 else int(rng.integers(1, n))
+        esz = int(rng.choice([4, 8, 16]))
+        R = W.removals(["uniform", "clustered", "tail_heavy"][i % 3], n, k, seed=seed + i)
+        lists.append((W.payload(esz, n, seed=i).cuda(), R.cuda()))
+    return lists
+
+
+def test_batched_choice_matches_forward(ctx):
+    """rz_dispatch_choose / rz_dispatch_collect (one call for many lists)
+    against the features and forward pass called separately: the same argmin
+    for every list with 0 < k < n, Redzone for k = 0 and k = n; twice in a row
+    with fewer lists the second time, with this dispatcher's parameters and
+    with the ones installed in the context (the committed model in both)."""
+    disp = D.Dispatcher.load(ctx)
+    assert disp.trained
+    for seed, count in ((90, 70), (91, 23)):
+        lists = _mixed_lists(seed, count)
+        descs = ctx.descs(lists)
+        announce("batched choice", seed, count)
+        want = disp.predict(disp.features(descs))[1].cpu().numpy()
+        n, k = descs.rows["n"], descs.rows["k"]
+        want = np.where((k == 0) | (k == n), D.RULE_REDZONE, want)
+        disp.choose_async(descs)
+        got = disp.collect()
+        assert np.array_equal(got, want), (got, want)
+        # null parameters: the model installed in the context
+        out = np.empty(count, np.int32)
+        ctx._call("rz_dispatch_choose", ctx.handle, None, descs.as_ctypes(), count, D._stream_ptr(None, ctx.device))
+        ctx._call("rz_dispatch_collect", ctx.handle, out.ctypes.data_as(D.ctypes.POINTER(D.ctypes.c_int32)), count)
+        assert np.array_equal(out, want)
+
+
+def test_batched_choice_untrained_uses_rule(ctx):
+    descs = ctx.descs(_mixed_lists(92, 30))
+    disp = D.Dispatcher(ctx)
+    disp.choose_async(descs)
+    n, k = descs.rows["n"], descs.rows["k"]
+    want = np.where((k == 0) | (k == n), D.RULE_REDZONE, np.where(2 * k < n, D.RULE_REDZONE, D.RULE_PARTITION))
+    assert np.array_equal(disp.collect(), want)
