@@ -455,7 +455,8 @@ struct rz_batch {
   uint32_t nitems = 0, grid = 0, chunk = 1;
   rz::QueueKernel kern = nullptr;
   rz::QueueSplit split;
-  rz::DeviceBuffer plans, items, qctl, bits, counts, orphans, slots, ents, curs;
+  rz::DeviceBuffer plans, qctl, bits, counts, orphans, slots, ents, curs;  // plans holds the items too
+  const uint2* d_items = nullptr;  // the items, in `plans` after the list plans
   size_t lds = 0;  // dynamic LDS of the launch: the longest fused-item slice's orphan list
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -616,9 +617,10 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
   }
   split.off[kQueues] = (uint32_t)items.size();
   rz_status st;
+  // the list plans and the items in one buffer, so that one copy uploads both
   const size_t plan_bytes = plans.size() * sizeof(ListPlan), item_bytes = items.size() * sizeof(uint2);
-  if ((st = ensure(ctx, b->plans, std::max<size_t>(64, plan_bytes), s)) != RZ_OK ||
-      (st = ensure(ctx, b->items, std::max<size_t>(8, item_bytes), s)) != RZ_OK ||
+  const size_t item_off = (plan_bytes + 255) & ~(size_t)255, up_bytes = item_off + item_bytes;
+  if ((st = ensure(ctx, b->plans, std::max<size_t>(64, up_bytes), s)) != RZ_OK ||
       (st = ensure(ctx, b->qctl, (size_t)(kQWords + 3 * (size_t)nlists + 64) * 4, s)) != RZ_OK ||
       (st = ensure(ctx, b->ents, std::max<size_t>(8, (size_t)ent_words * 4), s)) != RZ_OK ||
       (st = ensure(ctx, b->curs, std::max<size_t>(8, (size_t)cur_words * 4), s)) != RZ_OK ||
@@ -635,8 +637,9 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
       P.ent = (uint64_t)(uintptr_t)((uint32_t*)b->ents.ptr + P.ent);
       P.cur = (uint64_t)(uintptr_t)((uint32_t*)b->curs.ptr + P.cur);
     }
+  b->d_items = (const uint2*)((char*)b->plans.ptr + item_off);
   if (async) {
-    const size_t need = plan_bytes + item_bytes;
+    const size_t need = up_bytes;
     if (b->staged) RZ_HIP(hipEventSynchronize(b->staged_ev));  // the previous upload has left the staging area
     if (need > b->staging_bytes) {
       if (b->staging) RZ_HIP(hipHostFree(b->staging));
@@ -648,15 +651,13 @@ static rz_status batch_plan(rz_batch* b, const rz_list_desc* lists, int32_t nlis
     }
     if (!b->staged_ev) RZ_HIP(hipEventCreateWithFlags(&b->staged_ev, hipEventDisableTiming));
     memcpy(b->staging, plans.data(), plan_bytes);
-    memcpy((char*)b->staging + plan_bytes, items.data(), item_bytes);
-    if (plan_bytes) RZ_HIP(hipMemcpyAsync(b->plans.ptr, b->staging, plan_bytes, hipMemcpyHostToDevice, s));
-    if (item_bytes)
-      RZ_HIP(hipMemcpyAsync(b->items.ptr, (char*)b->staging + plan_bytes, item_bytes, hipMemcpyHostToDevice, s));
+    memcpy((char*)b->staging + item_off, items.data(), item_bytes);
+    if (up_bytes) RZ_HIP(hipMemcpyAsync(b->plans.ptr, b->staging, up_bytes, hipMemcpyHostToDevice, s));
     RZ_HIP(hipEventRecord(b->staged_ev, s));
     b->staged = true;
   } else {
     if (plan_bytes) RZ_HIP(hipMemcpy(b->plans.ptr, plans.data(), plan_bytes, hipMemcpyHostToDevice));
-    if (item_bytes) RZ_HIP(hipMemcpy(b->items.ptr, items.data(), item_bytes, hipMemcpyHostToDevice));
+    if (item_bytes) RZ_HIP(hipMemcpy((char*)b->plans.ptr + item_off, items.data(), item_bytes, hipMemcpyHostToDevice));
   }
   const size_t lds = (size_t)fused_L * 4;
   // some list runs fused items, or partition items
@@ -743,7 +744,7 @@ rz_status rz_batch_launch(rz_batch* b, void* stream) {
   }
   uint32_t* q = (uint32_t*)b->qctl.ptr;
   hipLaunchKernelGGL(b->kern, dim3(b->grid), dim3(b->cfg.wg_size), b->lds, s, ListPlan{},
-                     (const ListPlan*)b->plans.ptr, (const uint2*)b->items.ptr, b->split, b->chunk, q,
+                     (const ListPlan*)b->plans.ptr, b->d_items, b->split, b->chunk, q,
                      q + kQWords, (uint32_t)b->nlists, (uint32_t*)b->bits.ptr, (uint64_t*)b->counts.ptr,
                      (uint64_t*)b->slots.ptr, (uint32_t*)b->ctx->ctrl.ptr);
   RZ_HIP(hipGetLastError());
@@ -763,7 +764,7 @@ rz_status rz_batch_capture(rz_batch* b, void* stream) {
   }
   RZ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   hipLaunchKernelGGL(b->kern, dim3(b->grid), dim3(b->cfg.wg_size), b->lds, s, ListPlan{},
-                     (const ListPlan*)b->plans.ptr, (const uint2*)b->items.ptr, b->split, b->chunk,
+                     (const ListPlan*)b->plans.ptr, b->d_items, b->split, b->chunk,
                      (uint32_t*)b->qctl.ptr, (uint32_t*)b->qctl.ptr + kQWords, (uint32_t)b->nlists,
                      (uint32_t*)b->bits.ptr, (uint64_t*)b->counts.ptr, (uint64_t*)b->slots.ptr,
                      (uint32_t*)b->ctx->ctrl.ptr);
@@ -792,7 +793,6 @@ rz_status rz_batch_destroy(rz_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   free_buf(b->plans);
-  free_buf(b->items);
   free_buf(b->qctl);
   free_buf(b->bits);
   free_buf(b->counts);

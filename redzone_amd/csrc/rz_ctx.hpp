@@ -109,6 +109,7 @@ struct rz_ctx {
   uint8_t* auto_host = nullptr;  // pinned staging of the descriptor and the choice
   rz::DeviceBuffer choose;       // rz_dispatch_choose: descriptors, features, choices
   uint8_t* choose_host = nullptr;  // its pinned staging: descriptors, then choices
+  uint8_t* choose_host_dev = nullptr;  // the same staging's device address
   size_t choose_host_bytes = 0;
   hipEvent_t choose_ev = nullptr;  // recorded after the choices' copy back
   int32_t choose_n = -1;           // lists of the choice not yet collected (-1: none)

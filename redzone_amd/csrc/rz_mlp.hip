@@ -24,13 +24,10 @@ static_assert(NP == RZ_MLP_PARAMS, "parameter count");
 // ---------------------------------------------------------------------------
 // features: one wave per list, a 256-entry strided sample of the removal list
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void rz_features_kernel(const rz_list_desc* __restrict__ lists, int32_t nlists,
-                                                          float* __restrict__ out) {
-  const int li = blockIdx.x;
-  if (li >= nlists) return;
-  const rz_list_desc d = lists[li];
+// The features of list d, by one 64-lane workgroup (s_x: 256 floats of LDS);
+// lane 0 writes them to o[0..F).
+__device__ __forceinline__ void list_features(const rz_list_desc& d, float* s_x, float* o) {
   const int lane = threadIdx.x;
-  __shared__ float s_x[256];
   const double n = (double)(d.n > 0 ? d.n : 1);
   const int64_t k = d.k;
   const int64_t sz = d.n - d.k;
@@ -64,7 +61,6 @@ __global__ __launch_bounds__(64) void rz_features_kernel(const rz_list_desc* __r
   near = wave_sum(near);
   up = wave_sum(up);
   if (lane == 0) {
-    float* o = out + (size_t)li * F;
     const float fm = m > 0 ? (float)m : 1.f;
     const float mean = sum / fm;
     const float var = fmaxf(sum2 / fm - mean * mean, 0.f);
@@ -82,6 +78,14 @@ __global__ __launch_bounds__(64) void rz_features_kernel(const rz_list_desc* __r
     o[10] = m > 1 ? up / (float)(m - 1) : 0.f;
     o[11] = 1.f;
   }
+}
+
+__global__ __launch_bounds__(64) void rz_features_kernel(const rz_list_desc* __restrict__ lists, int32_t nlists,
+                                                          float* __restrict__ out) {
+  const int li = blockIdx.x;
+  if (li >= nlists) return;
+  __shared__ float s_x[256];
+  list_features(lists[li], s_x, out + (size_t)li * F);
 }
 
 // ---------------------------------------------------------------------------
@@ -115,22 +119,75 @@ __device__ __forceinline__ void forward(const float* __restrict__ P, const float
   }
 }
 
+// One layer by one wave: lane o's pre-activation b[o] + sum_i W[o][i] in_i,
+// in_i broadcast from lane i, summed in the order dense_relu uses (so the
+// result is the same bits as forward()).
+__device__ __forceinline__ float layer_wave(const float* __restrict__ W, const float* __restrict__ b, float in,
+                                            int nin, int o) {
+  float acc = b[o];
+  for (int i = 0; i < nin; ++i) acc = fmaf(W[o * nin + i], __shfl(in, i), acc);
+  return acc;
+}
+
+static_assert((H & (H - 1)) == 0 && (C & (C - 1)) == 0 && H <= 64 && F <= 64, "one unit per lane");
+
+// The network on one row by one 64-lane wave: lane f < F holds feature f on
+// entry; y_out (may be null) gets the C outputs.  Returns the argmin (every
+// lane), ties to the lower index as in rz_mlp_forward_kernel's contract.
+__device__ __forceinline__ int forward_wave(const float* __restrict__ P, float x, float* __restrict__ y_out) {
+  const int lane = threadIdx.x & 63, o = lane & (H - 1), c = lane & (C - 1);
+  float a = layer_wave(P + OW1, P + OB1, x, F, o);
+  float h = a > 0.f ? a : 0.f;
+  a = layer_wave(P + OW2, P + OB2, h, H, o);
+  h = a > 0.f ? a : 0.f;
+  a = layer_wave(P + OW3, P + OB3, h, H, o);
+  h = a > 0.f ? a : 0.f;
+  const float y = layer_wave(P + OW4, P + OB4, h, H, c);
+  if (y_out && lane < C) y_out[lane] = y;
+  int best = 0;
+  float yb = __shfl(y, 0);
+  for (int cc = 1; cc < C; ++cc) {
+    const float v = __shfl(y, cc);
+    if (v < yb) {
+      yb = v;
+      best = cc;
+    }
+  }
+  return best;
+}
+
+// One 64-lane workgroup per row.
 __global__ __launch_bounds__(64) void rz_mlp_forward_kernel(const float* __restrict__ params,
                                                              const float* __restrict__ X, int32_t B,
                                                              float* __restrict__ Y, int32_t* __restrict__ choice) {
-  __shared__ float P[NP];
-  for (int i = threadIdx.x; i < NP; i += blockDim.x) P[i] = params[i];
-  __syncthreads();
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.x;
   if (b >= B) return;
-  Acts t;
-  forward(P, X + (size_t)b * F, t);
-  int best = 0;
-  for (int c = 0; c < C; ++c) {
-    if (Y) Y[(size_t)b * C + c] = t.y[c];
-    if (t.y[c] < t.y[best]) best = c;
-  }
-  if (choice) choice[b] = best;
+  const int lane = threadIdx.x;
+  const float x = lane < F ? X[(size_t)b * F + lane] : 0.f;
+  const int best = forward_wave(params, x, Y ? Y + (size_t)b * C : nullptr);
+  if (choice && lane == 0) choice[b] = best;
+}
+
+// rz_dispatch_choose for up to kChooseArgMax lists: the descriptors come as a
+// kernel argument (no upload), one 64-lane workgroup per list computes its
+// features and the forward pass, and lane 0 stores the choice straight into
+// the caller-visible pinned staging with a system-scope store.
+constexpr int kChooseArgMax = 48;
+struct ChooseArgs {
+  rz_list_desc d[kChooseArgMax];
+};
+
+__global__ __launch_bounds__(64) void rz_choose_kernel(ChooseArgs a, int32_t nlists, const float* __restrict__ params,
+                                                        int32_t* __restrict__ h_choice) {
+  const int li = blockIdx.x;
+  if (li >= nlists) return;
+  __shared__ float s_x[256];
+  __shared__ float s_f[F];
+  list_features(a.d[li], s_x, s_f);
+  __syncthreads();
+  const float x = threadIdx.x < F ? s_f[threadIdx.x] : 0.f;
+  const int best = forward_wave(params, x, nullptr);
+  if (threadIdx.x == 0) __hip_atomic_store(&h_choice[li], best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Per-sample gradient of the masked MSE loss sum_b sum_c m_bc (y_bc - t_bc)^2 / sum(m)
@@ -242,8 +299,8 @@ rz_status rz_mlp_forward(rz_ctx* ctx, const float* d_params, const float* d_x, i
                          int32_t* d_choice, void* stream) {
   if (!ctx || !d_params || !d_x || batch < 0) return RZ_EINVAL;
   if (batch == 0) return RZ_OK;
-  hipLaunchKernelGGL(rz_mlp_forward_kernel, dim3((uint32_t)((batch + 63) / 64)), dim3(64), 0, (hipStream_t)stream,
-                     d_params, d_x, batch, d_y, d_choice);
+  hipLaunchKernelGGL(rz_mlp_forward_kernel, dim3((uint32_t)batch), dim3(64), 0, (hipStream_t)stream, d_params, d_x,
+                     batch, d_y, d_choice);
   RZ_HIP(hipGetLastError());
   return RZ_OK;
 }
@@ -373,13 +430,19 @@ rz_status rz_dispatch_choose(rz_ctx* ctx, const float* d_params, const rz_list_d
   const size_t cbytes = round256((size_t)nlists * 4 + 4);
   if (dbytes + cbytes > ctx->choose_host_bytes) {
     if (ctx->choose_host) RZ_HIP(hipHostFree(ctx->choose_host));
-    ctx->choose_host = nullptr;
+    ctx->choose_host = ctx->choose_host_dev = nullptr;
     ctx->choose_host_bytes = 0;
     size_t cap = 4096;
     while (cap < dbytes + cbytes) cap *= 2;
-    if (hipHostMalloc((void**)&ctx->choose_host, cap, hipHostMallocDefault) != hipSuccess) {
+    // mapped and coherent: the choice kernel stores into it directly
+    if (hipHostMalloc((void**)&ctx->choose_host, cap, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
       ctx->choose_host = nullptr;
       return RZ_ENOMEM;
+    }
+    if (hipHostGetDevicePointer((void**)&ctx->choose_host_dev, ctx->choose_host, 0) != hipSuccess) {
+      (void)hipHostFree(ctx->choose_host);
+      ctx->choose_host = ctx->choose_host_dev = nullptr;
+      return RZ_EHIP;
     }
     ctx->choose_host_bytes = cap;
   }
@@ -387,6 +450,18 @@ rz_status rz_dispatch_choose(rz_ctx* ctx, const float* d_params, const rz_list_d
   const float* params = d_params ? d_params : (ctx->dispatch_ready ? (const float*)ctx->dispatch.ptr : nullptr);
   if (!params || nlists == 0) {  // no network: the rule, on the host, at collection
     ctx->choose_dev = false;
+    ctx->choose_n = nlists;
+    return RZ_OK;
+  }
+  if (!ctx->choose_ev) RZ_HIP(hipEventCreateWithFlags(&ctx->choose_ev, hipEventDisableTiming));
+  if (nlists <= kChooseArgMax) {  // one launch, no copies
+    ChooseArgs a;
+    memcpy(a.d, h_lists, (size_t)nlists * sizeof(rz_list_desc));
+    hipLaunchKernelGGL(rz_choose_kernel, dim3((uint32_t)nlists), dim3(64), 0, s, a, nlists, params,
+                       (int32_t*)(ctx->choose_host_dev + dbytes));
+    RZ_HIP(hipGetLastError());
+    RZ_HIP(hipEventRecord(ctx->choose_ev, s));
+    ctx->choose_dev = true;
     ctx->choose_n = nlists;
     return RZ_OK;
   }
@@ -400,7 +475,6 @@ rz_status rz_dispatch_choose(rz_ctx* ctx, const float* d_params, const rz_list_d
     return st;
   RZ_HIP(hipMemcpyAsync(ctx->choose_host + dbytes, base + dbytes + fbytes, (size_t)nlists * 4, hipMemcpyDeviceToHost,
                         s));
-  if (!ctx->choose_ev) RZ_HIP(hipEventCreateWithFlags(&ctx->choose_ev, hipEventDisableTiming));
   RZ_HIP(hipEventRecord(ctx->choose_ev, s));
   ctx->choose_dev = true;
   ctx->choose_n = nlists;
