@@ -115,9 +115,19 @@ __device__ __forceinline__ void bucket_marks(const int32_t* __restrict__ R, uint
   const uint32_t nb = (k + kBucketW - 1) >> kBucketLog;
   for (uint32_t w = tid; w < nb; w += WG) s_cnt[w] = 0u;
   __syncthreads();
-  for (uint32_t i = a + tid; i < b; i += WG) {
-    const uint32_t z = (uint32_t)R[i] - sZ;  // wraps to >= 2^31 > k below the tail
-    if (z < k) atomicAdd(&s_cnt[z >> kBucketLog], 1u);
+  // U loads per thread in flight, then their uses (a use between them would
+  // wait for each load); z = R[i] - sZ wraps to >= 2^31 > k below the tail,
+  // and a slot past b loads sZ - 1, which maps to 0xffffffff
+  constexpr int U = 8;
+  for (uint32_t i0 = a + tid; i0 < b; i0 += WG * U) {
+    uint32_t z[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) z[u] = i0 + u * WG < b ? (uint32_t)R[i0 + u * WG] : sZ - 1u;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t zz = z[u] - sZ;
+      if (zz < k) atomicAdd(&s_cnt[zz >> kBucketLog], 1u);
+    }
   }
   __syncthreads();
   if (tid < kWave) {  // exclusive prefix over the buckets, two per lane
@@ -136,11 +146,17 @@ __device__ __forceinline__ void bucket_marks(const int32_t* __restrict__ R, uint
     s_cnt[w] = 0u;
   }
   __syncthreads();
-  for (uint32_t i = a + tid; i < b; i += WG) {  // the entries in LDS, bucket after bucket
-    const uint32_t z = (uint32_t)R[i] - sZ;
-    if (z < k) {
-      const uint32_t bz = z >> kBucketLog;
-      s_stage[s_lbase[bz] + atomicAdd(&s_cnt[bz], 1u)] = z;
+  for (uint32_t i0 = a + tid; i0 < b; i0 += WG * U) {  // the entries in LDS, bucket after bucket
+    uint32_t z[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) z[u] = i0 + u * WG < b ? (uint32_t)R[i0 + u * WG] : sZ - 1u;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t zz = z[u] - sZ;
+      if (zz < k) {
+        const uint32_t bz = zz >> kBucketLog;
+        s_stage[s_lbase[bz] + atomicAdd(&s_cnt[bz], 1u)] = zz;
+      }
     }
   }
   __syncthreads();
@@ -166,9 +182,16 @@ __device__ __forceinline__ void bucket_bits(uint32_t b, uint32_t k, const uint32
   const uint32_t c = min(ld_relaxed(&cur[b]), kBucketW);
   const uint32_t* e = ent + (size_t)b * kBucketW;
   const uint32_t lo = b * kBucketW;
-  for (uint32_t t = tid; t < c; t += WG) {
-    const uint32_t off = ld_relaxed(&e[t]) - lo;
-    if (off < kBucketW) atomicOr(&s_bm[off >> 5], 1u << (off & 31));
+  constexpr int U = 8;  // loads in flight per thread, then their uses
+  for (uint32_t t0 = tid; t0 < c; t0 += WG * U) {
+    uint32_t v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = t0 + u * WG < c ? ld_relaxed(&e[t0 + u * WG]) : lo - 1u;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t off = v[u] - lo;  // lo - 1 maps to 0xffffffff
+      if (off < kBucketW) atomicOr(&s_bm[off >> 5], 1u << (off & 31));
+    }
   }
   __syncthreads();
   const uint32_t w0 = lo >> 5, wend = (k + 31) >> 5;

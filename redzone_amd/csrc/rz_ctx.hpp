@@ -177,6 +177,13 @@ rz_status dense_moves(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, int64_t k,
                       uint32_t* bits, uint32_t* ctrl, int stats, bool bucket_marks, hipStream_t s);
 // dense form: whether it sets the tail marks itself (bucketed) instead of the atomic mark kernel
 bool dense_bucket_marks(const rz_tile_config& cfg, int64_t k, uint32_t flags);
+// Bucketed marks (rz_partition.hip): removals r in [base, base + m) set bit
+// r - base of the m-bit bitmap `bits` (zero on entry); epoch_ctrl (may be
+// null) gets the compaction's epoch started, stat (may be null) the number of
+// removals in the window.  m up to 2^29 (bucket_marks_fit).
+bool bucket_marks_fit(int64_t m);
+rz_status bucket_marks(rz_ctx* ctx, const int32_t* R, int64_t k, uint32_t base, int64_t m, uint32_t* bits,
+                       uint32_t* epoch_ctrl, uint64_t* stat, hipStream_t s);
 // dense form: whether (cfg, n, k) can be launched (checked before any kernel of a call runs)
 bool dense_shape_ok(const rz_ctx* ctx, const rz_tile_config& cfg, uint32_t esz, int64_t n, int64_t k, uint32_t flags);
 // After a failed device check: zero the scratch the kernels keep zero between

@@ -9,13 +9,9 @@
 // head writes by address before issuing them, in streaming passes:
 //
 //   tail marks (cfg.chunk == 2; otherwise the atomic rz_mark_tail)
-//     tcount    tile j of R: per tail bin (2^19 tail positions), the number
-//               of removals inside the red zone
-//     colscan / topscan   bin bases and per-tile offsets
-//     tentries  the tile's red-zone removals, staged in LDS by bin, written
-//               as contiguous runs
-//     tmark     one workgroup per tail bin sets the bin's bits in a 64 KiB
-//               LDS bitmap and writes its words whole
+//     bucket_marks (rz_partition.hip) over the window [sZ, n): the red-zone
+//               removals appended to per-bin entry lists, then each bin's
+//               bits set in an LDS bitmap and stored (two kernels)
 //   head writes
 //     count     tile j: per coarse destination bin, the number of head writes
 //               (moves and dest orphans) and the tile's dest / source orphan
@@ -40,9 +36,9 @@
 //               time and the writes combine in its L2: A[d] = value.
 //
 // Counts and offsets are recomputed on every call.  The fine cursors are
-// zeroed by the count kernel of the next call; the tail bitmap is overwritten
-// whole by tmark (or set by rz_mark_tail into a zero bitmap) and cleared by
-// the entries kernel as it reads it.  With an invalid list the passes still
+// zeroed by the count kernel of the next call; the tail bitmap is zero on
+// entry, set by the tail marks (or rz_mark_tail) and cleared by the entries
+// kernel as it reads it.  With an invalid list the passes still
 // classify identically, every index is checked against its buffer, and the
 // orphan-count mismatch (or a fine-bin overflow) is flagged.
 #include <atomic>
@@ -53,10 +49,7 @@ namespace rz {
 
 constexpr int kCoarseMax = 1024;      // coarse destination bins and tail bins
 constexpr int kFineMax = 4096;        // fine destination bins
-constexpr uint32_t kTailBinLog = 19;    // tail positions per tail bin of the dense form (a 64 KiB LDS bitmap)
-constexpr uint32_t kBucketBinLog = 19;  // positions per bin of bucket_marks (a 64 KiB LDS bitmap)
-// rz_dense_tail_mark's dynamic-LDS limit, set once for both of its users
-constexpr size_t kTailMarkLdsMax = (size_t)4 << (kBucketBinLog - 5);
+constexpr uint32_t kTailBinLog = 19;  // the tail marks' largest bin: k up to kCoarseMax << kTailBinLog = 2^29
 constexpr uint32_t kCoarseTarget = 64;
 
 __device__ __forceinline__ uint32_t classify(uint32_t d, bool dead, uint32_t sZ) {
@@ -434,134 +427,6 @@ __global__ __launch_bounds__(1024) void rz_dense_scatter(T* __restrict__ A, cons
   }
 }
 
-// ---- bucketed tail marks ------------------------------------------------------
-template <int WG, int IPT>
-// k removals; marked positions [sZ, sZ + m) (m = k for the red zone)
-__global__ __launch_bounds__(WG) void rz_dense_tail_count(const int32_t* __restrict__ R, uint32_t k, uint32_t sZ,
-                                                           uint32_t m, uint32_t L, uint32_t NBt, uint32_t blog,
-                                                           uint32_t* __restrict__ cnt) {
-  __shared__ uint32_t hist[kCoarseMax];
-  const uint32_t tid = threadIdx.x, j = blockIdx.x;
-  const uint32_t a = j * L, b = min(a + L, k);
-  for (uint32_t q = tid; q < NBt; q += WG) hist[q] = 0u;
-  __syncthreads();
-  for (uint32_t base = a; base < b; base += WG * IPT) {
-    uint32_t z[IPT];
-#pragma unroll
-    for (int q = 0; q < IPT; ++q) {
-      const uint32_t p = base + q * WG + tid;
-      z[q] = p < b ? (uint32_t)R[p] - sZ : 0xffffffffu;  // wraps to >= m below the window
-    }
-#pragma unroll
-    for (int q = 0; q < IPT; ++q) lds_bin_count(hist, z[q] >> blog, z[q] < m);
-  }
-  __syncthreads();
-  for (uint32_t q = tid; q < NBt; q += WG) cnt[(size_t)j * NBt + q] = hist[q];
-}
-
-// Dynamic LDS: S red-zone offsets and S bin ids (S = WG * IPT).
-template <int WG, int IPT>
-__global__ __launch_bounds__(WG) void rz_dense_tail_entries(const int32_t* __restrict__ R, uint32_t k, uint32_t sZ,
-                                                             uint32_t m, uint32_t L, uint32_t NBt, uint32_t blog,
-                                                             const uint32_t* __restrict__ off,
-                                                             const uint32_t* __restrict__ binbase,
-                                                             uint32_t* __restrict__ ent) {
-  constexpr uint32_t S = WG * IPT;
-  extern __shared__ __align__(16) unsigned char dyn[];
-  uint32_t* sz = (uint32_t*)dyn;
-  uint16_t* sb = (uint16_t*)(sz + S);
-  __shared__ uint32_t lcnt[kCoarseMax + 1], gb[kCoarseMax], gcur[kCoarseMax];
-  __shared__ uint32_t tmp[WG / kWave + 1];
-  const uint32_t tid = threadIdx.x, j = blockIdx.x;
-  const uint32_t a = j * L, b = min(a + L, k);
-  for (uint32_t q = tid; q < NBt; q += WG) {
-    gcur[q] = binbase[q] + off[(size_t)j * NBt + q];
-    lcnt[q] = 0u;
-  }
-  __syncthreads();
-  for (uint32_t base = a; base < b; base += S) {
-    uint32_t z[IPT], rk[IPT];
-#pragma unroll
-    for (int q = 0; q < IPT; ++q) {
-      const uint32_t p = base + q * WG + tid;
-      z[q] = p < b ? (uint32_t)R[p] - sZ : 0xffffffffu;
-    }
-#pragma unroll
-    for (int q = 0; q < IPT; ++q) rk[q] = lds_bin_add(lcnt, z[q] >> blog, z[q] < m);
-    __syncthreads();
-    for (uint32_t q = tid; q < NBt; q += WG) {
-      gb[q] = gcur[q];
-      gcur[q] += lcnt[q];
-    }
-    __syncthreads();
-    const uint32_t total = lds_excl_scan<WG>(lcnt, NBt, tmp);
-#pragma unroll
-    for (int q = 0; q < IPT; ++q) {
-      if (z[q] >= m) continue;
-      const uint32_t bin = z[q] >> blog;
-      const uint32_t e = lcnt[bin] + rk[q];
-      sz[e] = z[q];
-      sb[e] = (uint16_t)bin;
-    }
-    __syncthreads();
-    for (uint32_t e = tid; e < total; e += WG) {
-      const uint32_t bin = sb[e];
-      ent[gb[bin] + (e - lcnt[bin])] = sz[e];
-    }
-    __syncthreads();
-    for (uint32_t q = tid; q < NBt; q += WG) lcnt[q] = 0u;
-    __syncthreads();
-  }
-}
-
-// G workgroups per tail bin (workgroup g of bin b is block b*G + g): each
-// sets the bits of its share of the bin's entries in a dynamic-LDS bitmap of
-// 2^blog bits.  With G = 1 it writes the bin's words whole (every word
-// of the m-bit bitmap belongs to exactly one bin, so the bitmap need not be
-// clear beforehand); with G > 1 it ORs its nonzero words in (the bitmap must
-// be zero beforehand).
-__global__ __launch_bounds__(1024) void rz_dense_tail_mark(const uint32_t* __restrict__ ent,
-                                                          const uint32_t* __restrict__ binbase, uint32_t m,
-                                                          uint32_t* __restrict__ bits,
-                                                          uint64_t* __restrict__ stat_tail, uint32_t NBt,
-                                                          uint32_t G, uint32_t blog) {
-  const uint32_t W = 1u << (blog - 5);
-  extern __shared__ __align__(16) unsigned char dyn[];
-  uint32_t* lb = (uint32_t*)dyn;
-  const uint32_t bin = blockIdx.x / G, g = blockIdx.x % G, tid = threadIdx.x;
-  for (uint32_t w = tid; w < W; w += 1024u) lb[w] = 0u;
-  __syncthreads();
-  const uint32_t z0 = bin << blog;
-  const uint32_t s0 = binbase[bin], e0 = binbase[bin + 1];
-  const uint32_t share = (e0 - s0 + G - 1) / G;
-  const uint32_t s = s0 + min(g * share, e0 - s0), e = s0 + min((g + 1) * share, e0 - s0);
-  // 8 loads in flight per thread, then their LDS atomics
-  constexpr int U = 8;
-  for (uint32_t q0 = s + tid; q0 < e; q0 += 1024u * U) {
-    uint32_t r[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t q = q0 + u * 1024u;
-      r[u] = q < e ? ent[q] - z0 : 0xffffffffu;  // < 2^blog for an entry of this bin
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (r[u] < (1u << blog)) atomicOr(&lb[r[u] >> 5], 1u << (r[u] & 31));
-  }
-  __syncthreads();
-  const uint32_t nw = (uint32_t)(((uint64_t)m + 31u) >> 5);
-  for (uint32_t w = tid; w < W; w += 1024u) {
-    const uint32_t gw = (z0 >> 5) + w;
-    if (gw < nw) {
-      if (G == 1)
-        bits[gw] = lb[w];
-      else if (lb[w])
-        or_relaxed(&bits[gw], lb[w]);
-    }
-  }
-  if (stat_tail && blockIdx.x == 0 && tid == 0) *stat_tail = binbase[NBt];
-}
-
 // ---------------------------------------------------------------------------
 // launch
 // ---------------------------------------------------------------------------
@@ -599,18 +464,12 @@ static rz_status dense_launch(rz_ctx* ctx, T* A, const int32_t* R, uint32_t k, u
                               uint32_t* bits, uint32_t* ctrl, int stats, uint32_t src_cap, hipStream_t s) {
   constexpr uint32_t S = WG * IPT;
   const size_t lds_ent = (size_t)S * (sizeof(T) + 4 + 2);
-  const size_t lds_tail = (size_t)S * (4 + 2);
-  const size_t lds_mark = (size_t)4 << (kTailBinLog - 5);
   static std::atomic<uint64_t> attr_mask{0};
   if (attr_needed(attr_mask, ctx->device)) {
     RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_entries<T, WG, IPT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds_ent));
     RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_refine<T, WG, IPT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds_ent));
-    RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_tail_entries<WG, IPT>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_tail));
-    RZ_HIP(hipFuncSetAttribute((const void*)rz_dense_tail_mark, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)kTailMarkLdsMax));
     attr_done(attr_mask, ctx->device);
   }
   uint32_t* cnt = (uint32_t*)ctx->dense_cnt.ptr;
@@ -629,21 +488,11 @@ static rz_status dense_launch(rz_ctx* ctx, T* A, const int32_t* R, uint32_t k, u
   T* src = (T*)ctx->dense_src.ptr;
   uint32_t* dpos = (uint32_t*)((char*)ctx->dense_src.ptr + (((size_t)src_cap * sizeof(T) + 255) & ~(size_t)255));
   if (d.NBt) {
-    // the red-zone offsets use ent_d's space before the head writes do
-    uint32_t* toff = cnt + (size_t)d.nt * d.NBt;
-    hipLaunchKernelGGL((rz_dense_tail_count<WG, IPT>), dim3(d.nt), dim3(WG), 0, s, R, k, sZ, k, d.L, d.NBt,
-                       kTailBinLog, cnt);
-    hipLaunchKernelGGL(rz_dense_colscan, dim3(d.NBt / 32), dim3(1024), 0, s, cnt, d.nt, d.NBt, toff, tbtot);
-    hipLaunchKernelGGL(rz_dense_topscan, dim3(1), dim3(1024), 0, s, tbtot, d.NBt, tbinbase, ocnt, 0u,
-                       (uint64_t*)nullptr, ctrl, 0);
-    hipLaunchKernelGGL((rz_dense_tail_entries<WG, IPT>), dim3(d.nt), dim3(WG), lds_tail, s, R, k, sZ, k, d.L,
-                       d.NBt, kTailBinLog, toff, tbinbase, ent_d);
-    // the tail bitmap is zero here (every launch form leaves it clear), so
-    // several workgroups can share a bin and OR their words in
-    uint32_t G = (2u * (uint32_t)ctx->cu_count + d.NBt - 1) / d.NBt;
-    G = G < 1 ? 1 : (G > 16 ? 16 : G);
-    hipLaunchKernelGGL(rz_dense_tail_mark, dim3(d.NBt * G), dim3(1024), lds_mark, s, ent_d, tbinbase, k, bits,
-                       stats ? (uint64_t*)(ctrl + kCtrlStatTail) : (uint64_t*)nullptr, d.NBt, G, kTailBinLog);
+    // tail marks grouped by tail region (the tail bitmap is zero here: every
+    // launch form leaves it clear)
+    rz_status st = bucket_marks(ctx, R, k, sZ, k, bits, nullptr,
+                                stats ? (uint64_t*)(ctrl + kCtrlStatTail) : (uint64_t*)nullptr, s);
+    if (st != RZ_OK) return st;
   }
   uint32_t* off = cnt + (size_t)d.nt * d.NB;
   hipLaunchKernelGGL((rz_dense_count<WG, IPT>), dim3(d.nt), dim3(WG), 0, s, R, k, sZ, d.L, bits, d.NB, d.bs, cnt,

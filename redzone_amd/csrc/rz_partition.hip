@@ -368,18 +368,22 @@ rz_status launch_validate(rz_ctx* ctx, const int32_t* R, int64_t n, int64_t k, u
 // ---------------------------------------------------------------------------
 constexpr int kPartWG = 256, kPartIPT = 16;
 // ---------------------------------------------------------------------------
-// Bucketed marks of the partition, two kernels, over bins of 2^blog
-// positions (2^15 to 2^19, so that a list has 256 to 1,024 bins).
+// Bucketed marks, two kernels: the removals r in a window [base, base + m)
+// set bit r - base of an m-bit bitmap that is zero on entry (the partition's
+// n-bit bitmap: base 0, m = n; the dense form's tail bitmap: base n - k,
+// m = k).  Bins of 2^blog positions (2^15 to 2^19, so that a window has 256 to
+// 1,024 bins).
 // rz_part_append: a workgroup takes 1,024 x IPT removals, counts them per bin in
 // LDS, reserves room in each bin's entry list (2^blog entries, a bin's
 // positions) with one atomic per bin, sorts them by bin in LDS and writes them
 // out bin by bin, so consecutive threads write consecutive words.
 // rz_part_bits: G workgroups per bin (G = 1 from 256 bins up) set the bin's
-// bits in an LDS bitmap and store its words into the list bitmap (zero on
-// entry; OR-ed when G > 1); the last of a bin's workgroups puts the bin's
-// count back to zero, and workgroup 0 starts the compaction's epoch.  No scans
-// and no epoch launch: two launches where the count / scan / entries / bits
-// pipeline took five and the epoch a sixth.
+// bits in an LDS bitmap and store its words into the bitmap (OR-ed when
+// G > 1); the last of a bin's workgroups puts the bin's count back to zero.
+// Optionally workgroup 0 starts the compaction's epoch, and the window's
+// removal count is summed into a stat word (zeroed by the append kernel).
+// No scans and no epoch launch: two launches where a count / scan / entries /
+// bits pipeline took five.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kPartMinBinLog = 15, kPartMaxBinLog = 19;  // 4 to 64 KiB LDS bitmaps
 constexpr uint32_t kPartMaxBins = 1024;                       // lists of up to 2^29 items
@@ -389,9 +393,10 @@ constexpr uint32_t kAppWG = 1024;
 constexpr int kAppIPT = 16, kAppIPTSmall = 8;
 
 template <int IPT>
-__global__ __launch_bounds__(1024) void rz_part_append(const int32_t* __restrict__ R, uint32_t k, uint32_t n,
-                                                        uint32_t blog, uint32_t nbins, uint32_t* __restrict__ ent,
-                                                        uint32_t* __restrict__ cur) {
+__global__ __launch_bounds__(1024) void rz_part_append(const int32_t* __restrict__ R, uint32_t k, uint32_t base,
+                                                        uint32_t m, uint32_t blog, uint32_t nbins,
+                                                        uint32_t* __restrict__ ent, uint32_t* __restrict__ cur,
+                                                        uint64_t* __restrict__ stat) {
   constexpr uint32_t kAppS = kAppWG * IPT;
   __shared__ uint32_t s_cnt[kPartMaxBins], s_lbase[kPartMaxBins], s_gbase[kPartMaxBins];
   __shared__ uint32_t s_tmp[kAppWG / kWave + 1];
@@ -400,33 +405,44 @@ __global__ __launch_bounds__(1024) void rz_part_append(const int32_t* __restrict
   const uint32_t tid = threadIdx.x;
   const uint32_t a = blockIdx.x * kAppS, b = min(a + kAppS, k);
   for (uint32_t w = tid; w < nbins; w += kAppWG) s_cnt[w] = 0u;
+  if (stat && blockIdx.x == 0 && tid == 0) st_relaxed(stat, (uint64_t)0);
   __syncthreads();
   uint32_t r[IPT];
 #pragma unroll
-  for (int j = 0; j < IPT; ++j) {
+  for (int j = 0; j < IPT; ++j) {  // all loads first (a use inside this loop would wait for each one)
     const uint32_t i = a + j * kAppWG + tid;
     r[j] = i < b ? (uint32_t)R[i] : 0xffffffffu;
   }
+  // positions in the window; 0xffffffff (>= m) for none (0xffffffff - base >= 2^31 > m)
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t p = r[j] - base;
+    r[j] = p < m ? p : 0xffffffffu;
+  }
   // a wave whose removals share one bin (a run) counts them with one atomic
 #pragma unroll
-  for (int j = 0; j < IPT; ++j) lds_bin_count(s_cnt, r[j] >> blog, r[j] < n);
+  for (int j = 0; j < IPT; ++j) lds_bin_count(s_cnt, r[j] >> blog, r[j] < m);
   __syncthreads();
   // exclusive prefix over the bins (one bin per thread), then room in each bin's list
   uint32_t total;
   const uint32_t c = tid < nbins ? s_cnt[tid] : 0u;
   const uint32_t ex = block_excl_scan<kAppWG>(c, s_tmp, &total);
+  // the reservation's atomics stay in flight while the removals are staged:
+  // only the write-out below needs their results
+  uint32_t gb = 0u;
   if (tid < nbins) {
     s_lbase[tid] = ex;
-    s_gbase[tid] = c ? add_relaxed(&cur[tid], c) : 0u;
+    if (c) gb = add_relaxed(&cur[tid], c);
     s_cnt[tid] = 0u;
   }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const uint32_t bz = r[j] >> blog;
-    const uint32_t rank = lds_bin_add(s_cnt, bz, r[j] < n);
-    if (r[j] < n) s_stage[s_lbase[bz] + rank] = r[j];
+    const uint32_t rank = lds_bin_add(s_cnt, bz, r[j] < m);
+    if (r[j] < m) s_stage[s_lbase[bz] + rank] = r[j];
   }
+  if (tid < nbins) s_gbase[tid] = gb;
   __syncthreads();
   for (uint32_t t = tid; t < total; t += kAppWG) {
     const uint32_t v = s_stage[t];
@@ -437,9 +453,9 @@ __global__ __launch_bounds__(1024) void rz_part_append(const int32_t* __restrict
 }
 
 __global__ __launch_bounds__(1024) void rz_part_bits(const uint32_t* __restrict__ ent, uint32_t* __restrict__ cur,
-                                                      uint32_t* __restrict__ bdone, uint32_t n, uint32_t blog,
+                                                      uint32_t* __restrict__ bdone, uint32_t m, uint32_t blog,
                                                       uint32_t G, uint32_t* __restrict__ bits,
-                                                      uint32_t* __restrict__ ctrl) {
+                                                      uint32_t* __restrict__ ctrl, uint64_t* __restrict__ stat) {
   const uint32_t W = 1u << (blog - 5), cap = 1u << blog;
   extern __shared__ __align__(16) unsigned char dyn[];
   uint32_t* lb = (uint32_t*)dyn;
@@ -458,26 +474,27 @@ __global__ __launch_bounds__(1024) void rz_part_bits(const uint32_t* __restrict_
   constexpr int U = 16;
   for (uint32_t q0 = s + tid * U; q0 < e; q0 += 1024u * U) {
     uint32_t r[U];
+    if (q0 + U <= e) {  // all U entries: U/4 vector loads, issued together (no branch between them)
+      uint4 x[U / 4];
 #pragma unroll
-    for (int v = 0; v < U / 4; ++v) {
-      const uint32_t q = q0 + 4u * v;
-      if (q + 4u <= e) {
-        const uint4 x = *(const uint4*)(be + q);
-        r[4 * v] = x.x - z0;
-        r[4 * v + 1] = x.y - z0;
-        r[4 * v + 2] = x.z - z0;
-        r[4 * v + 3] = x.w - z0;
-      } else {
+      for (int v = 0; v < U / 4; ++v) x[v] = *(const uint4*)(be + q0 + 4u * v);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) r[4 * v + i] = q + i < e ? be[q + i] - z0 : 0xffffffffu;
+      for (int v = 0; v < U / 4; ++v) {
+        r[4 * v] = x[v].x - z0;
+        r[4 * v + 1] = x[v].y - z0;
+        r[4 * v + 2] = x[v].z - z0;
+        r[4 * v + 3] = x[v].w - z0;
       }
+    } else {  // the end of the bin's share
+#pragma unroll
+      for (int i = 0; i < U; ++i) r[i] = q0 + i < e ? be[q0 + i] - z0 : 0xffffffffu;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
       if (r[u] < cap) atomicOr(&lb[r[u] >> 5], 1u << (r[u] & 31));
   }
   __syncthreads();
-  const uint32_t nw = (n + 31u) >> 5;
+  const uint32_t nw = (uint32_t)(((uint64_t)m + 31u) >> 5);
   for (uint32_t w = tid; w < W; w += 1024u) {
     const uint32_t gw = (z0 >> 5) + w;
     if (gw < nw && lb[w]) {
@@ -491,8 +508,51 @@ __global__ __launch_bounds__(1024) void rz_part_bits(const uint32_t* __restrict_
       st_relaxed(&cur[bin], 0u);
       st_relaxed(&bdone[bin], 0u);
     }
-    if (blockIdx.x == 0) add_relaxed(&ctrl[kCtrlEpoch], 1u);
+    if (stat && g == 0 && c) atomicAdd((unsigned long long*)stat, (unsigned long long)c);
+    if (ctrl && blockIdx.x == 0) add_relaxed(&ctrl[kCtrlEpoch], 1u);
   }
+}
+
+bool bucket_marks_fit(int64_t m) { return m > 0 && m <= ((int64_t)kPartMaxBins << kPartMaxBinLog); }
+
+rz_status bucket_marks(rz_ctx* ctx, const int32_t* R, int64_t k, uint32_t base, int64_t m, uint32_t* bits,
+                       uint32_t* epoch_ctrl, uint64_t* stat, hipStream_t s) {
+  if (!bucket_marks_fit(m) || k < 1 || k > INT32_MAX) return RZ_EINVAL;
+  uint32_t blog = kPartMinBinLog;
+  while (blog < kPartMaxBinLog && (m >> blog) > 512) ++blog;
+  const uint32_t nbins = (uint32_t)((m + (1ll << blog) - 1) >> blog);
+  rz_status st;
+  if ((st = ensure(ctx, ctx->pent, (size_t)nbins << (blog + 2), s)) != RZ_OK) return st;
+  if ((st = ensure(ctx, ctx->pcur, (size_t)nbins * 8, s)) != RZ_OK) return st;  // counts, then done counters
+  constexpr size_t lds_max = (size_t)4 << (kPartMaxBinLog - 5);
+  const size_t lds_bits = (size_t)4 << (blog - 5);
+  // the big stage where the device gives a workgroup its LDS (plus about 12 KiB static)
+  constexpr size_t lds_app = (size_t)4 * kAppWG * kAppIPT, lds_app_small = (size_t)4 * kAppWG * kAppIPTSmall;
+  const bool big = lds_app + 16384 <= (size_t)ctx->lds_max;
+  static std::atomic<uint64_t> attr_mask{0};
+  const int dev = ctx->device;
+  if (dev < 0 || dev >= 64 || !(attr_mask.load(std::memory_order_relaxed) & (1ull << dev))) {
+    RZ_HIP(hipFuncSetAttribute((const void*)rz_part_bits, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
+    if (big)
+      RZ_HIP(hipFuncSetAttribute((const void*)rz_part_append<kAppIPT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds_app));
+    if (dev >= 0 && dev < 64) attr_mask.fetch_or(1ull << dev, std::memory_order_relaxed);
+  }
+  uint32_t* cur = (uint32_t*)ctx->pcur.ptr;
+  uint32_t* ent = (uint32_t*)ctx->pent.ptr;
+  if (big)
+    hipLaunchKernelGGL(rz_part_append<kAppIPT>, dim3((uint32_t)((k + kAppWG * kAppIPT - 1) / (kAppWG * kAppIPT))),
+                       dim3(kAppWG), lds_app, s, R, (uint32_t)k, base, (uint32_t)m, blog, nbins, ent, cur, stat);
+  else
+    hipLaunchKernelGGL(rz_part_append<kAppIPTSmall>,
+                       dim3((uint32_t)((k + kAppWG * kAppIPTSmall - 1) / (kAppWG * kAppIPTSmall))), dim3(kAppWG),
+                       lds_app_small, s, R, (uint32_t)k, base, (uint32_t)m, blog, nbins, ent, cur, stat);
+  uint32_t G = ((uint32_t)ctx->cu_count + nbins - 1) / nbins;  // about one workgroup per CU
+  G = G < 1 ? 1 : (G > 16 ? 16 : G);
+  hipLaunchKernelGGL(rz_part_bits, dim3(nbins * G), dim3(1024), lds_bits, s, (const uint32_t*)ent, cur, cur + nbins,
+                     (uint32_t)m, blog, G, bits, epoch_ctrl, stat);
+  RZ_HIP(hipGetLastError());
+  return RZ_OK;
 }
 
 // Marks of lists of at least kPartBucketMinN items and kPartBucketMinK removals
@@ -533,41 +593,10 @@ static rz_status partition_impl(rz_ctx* ctx, T* A, int64_t n, const int32_t* R, 
   const int64_t gw = (k + 255) / 256;
   const dim3 g1((uint32_t)(gw < (int64_t)ctx->cu_count * 8 ? (gw > 0 ? gw : 1) : ctx->cu_count * 8));
   if ((st = prof_mark(ctx, flags, 0, s)) != RZ_OK) return st;
-  const bool bucketed = k >= kPartBucketMinK && n >= kPartBucketMinN && n <= ((int64_t)kPartMaxBins << kPartMaxBinLog);
+  const bool bucketed = k >= kPartBucketMinK && n >= kPartBucketMinN && bucket_marks_fit(n);
   if (bucketed) {
     // the bitmap is zero here: every call leaves the words it used clear
-    uint32_t blog = kPartMinBinLog;
-    while (blog < kPartMaxBinLog && (n >> blog) > 512) ++blog;
-    const uint32_t nbins = (uint32_t)((n + (1ll << blog) - 1) >> blog);
-    if ((st = ensure(ctx, ctx->pent, (size_t)nbins << (blog + 2), s)) != RZ_OK) return st;
-    if ((st = ensure(ctx, ctx->pcur, (size_t)nbins * 8, s)) != RZ_OK) return st;  // counts, then done counters
-    constexpr size_t lds_max = (size_t)4 << (kPartMaxBinLog - 5);
-    const size_t lds_bits = (size_t)4 << (blog - 5);
-    // the big stage where the device gives a workgroup its LDS (plus about 12 KiB static)
-    constexpr size_t lds_app = (size_t)4 * kAppWG * kAppIPT, lds_app_small = (size_t)4 * kAppWG * kAppIPTSmall;
-    const bool big = lds_app + 16384 <= (size_t)ctx->lds_max;
-    static std::atomic<uint64_t> attr_mask{0};
-    const int dev = ctx->device;
-    if (dev < 0 || dev >= 64 || !(attr_mask.load(std::memory_order_relaxed) & (1ull << dev))) {
-      RZ_HIP(hipFuncSetAttribute((const void*)rz_part_bits, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
-      if (big)
-        RZ_HIP(hipFuncSetAttribute((const void*)rz_part_append<kAppIPT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds_app));
-      if (dev >= 0 && dev < 64) attr_mask.fetch_or(1ull << dev, std::memory_order_relaxed);
-    }
-    uint32_t* cur = (uint32_t*)ctx->pcur.ptr;
-    if (big)
-      hipLaunchKernelGGL(rz_part_append<kAppIPT>, dim3((uint32_t)((k + kAppWG * kAppIPT - 1) / (kAppWG * kAppIPT))),
-                         dim3(kAppWG), lds_app, s, R, (uint32_t)k, (uint32_t)n, blog, nbins, (uint32_t*)ctx->pent.ptr,
-                         cur);
-    else
-      hipLaunchKernelGGL(rz_part_append<kAppIPTSmall>,
-                         dim3((uint32_t)((k + kAppWG * kAppIPTSmall - 1) / (kAppWG * kAppIPTSmall))), dim3(kAppWG),
-                         lds_app_small, s, R, (uint32_t)k, (uint32_t)n, blog, nbins, (uint32_t*)ctx->pent.ptr, cur);
-    uint32_t G = ((uint32_t)ctx->cu_count + nbins - 1) / nbins;  // about one workgroup per CU
-    G = G < 1 ? 1 : (G > 16 ? 16 : G);
-    hipLaunchKernelGGL(rz_part_bits, dim3(nbins * G), dim3(1024), lds_bits, s, (const uint32_t*)ctx->pent.ptr, cur,
-                       cur + nbins, (uint32_t)n, blog, G, bits, ctrl);
+    if ((st = bucket_marks(ctx, R, k, 0u, n, bits, ctrl, nullptr, s)) != RZ_OK) return st;
   } else {
     hipLaunchKernelGGL(rz_mark_all, g1, dim3(256), 0, s, R, (uint32_t)k, (uint32_t)n, bits, ctrl);
   }
