@@ -149,7 +149,8 @@ def sweep_gather(ctx, rz, W, torch, reps, emit):
 
 def sweep_partition(ctx, rz, W, torch, reps, emit):
     """The O(n) stable partition alone on n = 2^28 uint32 with uniform removals
-    at k/n = 0.3% (atomic marks) and 50% (bucketed marks), beside Redzone."""
+    at k/n = 0.3% and 50% (both with bucketed marks: 2^19 removals and more),
+    beside Redzone."""
     n = 1 << 28
     time_call = _timer(torch)
     A0 = W.payload(4, n, seed=2026, device="cuda")

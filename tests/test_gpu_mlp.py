@@ -100,12 +100,14 @@ def _mixed_lists(seed, count):
 def test_batched_choice_matches_forward(ctx):
     """rz_dispatch_choose / rz_dispatch_collect (one call for many lists)
     against the features and forward pass called separately: the same argmin
-    for every list with 0 < k < n, Redzone for k = 0 and k = n; twice in a row
-    with fewer lists the second time, with this dispatcher's parameters and
-    with the ones installed in the context (the committed model in both)."""
+    for every list with 0 < k < n, Redzone for k = 0 and k = n; several calls
+    in a row, either side of the 48 lists that one kernel takes as its
+    argument (above, the descriptors are uploaded), with this dispatcher's
+    parameters and with the ones installed in the context (the committed
+    model in both)."""
     disp = D.Dispatcher.load(ctx)
     assert disp.trained
-    for seed, count in ((90, 70), (91, 23)):
+    for seed, count in ((90, 70), (91, 23), (93, 48), (94, 49)):  # 48: the most lists passed as a kernel argument
         lists = _mixed_lists(seed, count)
         descs = ctx.descs(lists)
         announce("batched choice", seed, count)
