@@ -399,6 +399,21 @@ rz_status rz_remove_auto(rz_ctx* ctx, void* d_data, int64_t n, uint32_t elem_byt
 rz_status rz_dispatch_choose(rz_ctx* ctx, const float* d_params, const rz_list_desc* h_lists, int32_t nlists,
                              void* stream);
 rz_status rz_dispatch_collect(rz_ctx* ctx, int32_t* h_choice, int32_t nlists);
+/* Many lists of any sizes with the method chosen per list (no counterpart in
+ * the reference; it generalises rz_remove_auto to a stream of lists): lists
+ * with fewer than batch_max_k removals run in one persistent batch
+ * (rz_remove_batched, their method fields ignored), the others each get their
+ * own launch with the network's method.  In stream order: the choice for the
+ * large lists is issued first (rz_dispatch_choose with d_params, or the
+ * installed network when null, or the k < n/2 rule without either), then the
+ * batch is planned and launched while the device computes it, then the
+ * choice is collected (waiting for that choice only) and the large lists are
+ * queued behind the batch.  methods (may be null) gets the RZ_METHOD_* that
+ * ran per list (RZ_METHOD_REDZONE for the batched ones).  flags apply to
+ * every call; the first failing call's status is returned.  The mixed-stream
+ * measurement runs this with batch_max_k = 2^22. */
+rz_status rz_remove_auto_batched(rz_ctx* ctx, const float* d_params, const rz_list_desc* lists, int32_t nlists,
+                                 int64_t batch_max_k, uint32_t flags, int32_t* methods, void* stream);
 
 /* ---- allocation-free removal -------------------------------------------- */
 /* No counterpart in the reference.  For callers that cannot allocate device

@@ -142,6 +142,7 @@ SIGNATURES = [
     ("rz_remove_auto", _i32, [_vp, _vp, _i64, _u32, _vp, _i64, _u32, ctypes.POINTER(_i32), _vp]),
     ("rz_dispatch_choose", _i32, [_vp, _vp, ctypes.POINTER(ListDesc), _i32, _vp]),
     ("rz_dispatch_collect", _i32, [_vp, ctypes.POINTER(_i32), _i32]),
+    ("rz_remove_auto_batched", _i32, [_vp, _vp, ctypes.POINTER(ListDesc), _i32, _i64, _u32, ctypes.POINTER(_i32), _vp]),
     ("rz_check_errors", _i32, [_vp, _vp, ctypes.POINTER(_u32)]),
     ("rz_debug_set_waits", _i32, [_vp, _u32, _u32]),
     ("rz_remove_noalloc", _i32, [_vp, _i64, _u32, _vp, _i64, _u32, _vp, _vp]),

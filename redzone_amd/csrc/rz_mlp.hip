@@ -10,6 +10,7 @@
 // summed in a fixed order, so a gradient is the same bits on every run.
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "rz_ctx.hpp"
 #include "rz_device.hpp"
@@ -497,6 +498,49 @@ rz_status rz_dispatch_collect(rz_ctx* ctx, int32_t* h_choice, int32_t nlists) {
     h_choice[i] = m;
   }
   ctx->choose_n = -1;
+  return RZ_OK;
+}
+
+rz_status rz_remove_auto_batched(rz_ctx* ctx, const float* d_params, const rz_list_desc* lists, int32_t nlists,
+                                 int64_t batch_max_k, uint32_t flags, int32_t* methods, void* stream) {
+  if (!ctx || nlists < 0 || (nlists > 0 && !lists) || nlists >= (1 << 28) || batch_max_k < 0) return RZ_EINVAL;
+  std::vector<rz_list_desc> small, large;
+  std::vector<int32_t> large_idx;
+  small.reserve((size_t)nlists);
+  for (int32_t i = 0; i < nlists; ++i) {
+    rz_list_desc d = lists[i];
+    d.method = 0;
+    if (d.k < batch_max_k) {
+      small.push_back(d);
+    } else {
+      large.push_back(d);
+      large_idx.push_back(i);
+    }
+    if (methods) methods[i] = RZ_METHOD_REDZONE;
+  }
+  rz_status st;
+  const int32_t nl = (int32_t)large.size();
+  // the choice first: the device computes it while the batch below is planned
+  if (nl && (st = rz_dispatch_choose(ctx, d_params, large.data(), nl, stream)) != RZ_OK) return st;
+  if (!small.empty() && (st = rz_remove_batched(ctx, small.data(), (int32_t)small.size(), flags, stream)) != RZ_OK)
+    return st;
+  if (!nl) return RZ_OK;
+  std::vector<int32_t> m((size_t)nl);
+  if ((st = rz_dispatch_collect(ctx, m.data(), nl)) != RZ_OK) return st;
+  for (int32_t j = 0; j < nl; ++j) {  // queued behind the batch on the stream
+    const rz_list_desc& d = large[(size_t)j];
+    if (m[j] == RZ_METHOD_PARTITION) {
+      st = rz_partition_on(ctx, d.data, d.n, d.elem_bytes, d.removals, d.k, flags, stream);
+    } else if (m[j] == RZ_METHOD_REDZONE) {
+      st = rz_remove(ctx, d.data, d.n, d.elem_bytes, d.removals, d.k, flags, stream);
+    } else {
+      rz_tile_config cfg;
+      if ((st = rz_dispatch_method_config(m[j], d.elem_bytes, &cfg)) == RZ_OK)
+        st = rz_remove_config(ctx, d.data, d.n, d.elem_bytes, d.removals, d.k, flags, &cfg, stream);
+    }
+    if (st != RZ_OK) return st;
+    if (methods) methods[large_idx[(size_t)j]] = m[j];
+  }
   return RZ_OK;
 }
 

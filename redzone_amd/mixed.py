@@ -209,7 +209,10 @@ def run_batched_learned(ctx, disp, lists, specs=None, batch_max_k=BATCH_MAX_K, b
     the copy back of the choices, in stream order); the batch is then planned
     and launched, and while the host plans, the device computes the choice, so
     that collecting it waits for nothing and the large lists queue up behind
-    the batch.  (Measured on the 10,000-list stream: deciding after the
+    the batch.  With a trained network the whole sequence is one library call
+    (rz_remove_auto_batched), as a native caller would make it; an untrained
+    dispatcher (the k < n/2 rule) and batch_partition take the same steps from
+    Python.  (Measured on the 10,000-list stream: deciding after the
     batch's launch left the device idle between the batch and the large
     lists, because the persistent batch holds every CU until it ends, and
     deciding first through separate feature and forward calls from Python
@@ -238,6 +241,17 @@ def run_batched_learned(ctx, disp, lists, specs=None, batch_max_k=BATCH_MAX_K, b
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     decide_small = batch_partition and disp.trained and len(small) > 0
+    if disp.trained and not decide_small:  # the whole sequence in one library call
+        from . import RedzoneError
+        from .dispatch import _ptr
+
+        st = ctx._lib.rz_remove_auto_batched(ctx.handle, _ptr(disp.params), descs.as_ctypes(), len(descs),
+                                             int(batch_max_k), 0,
+                                             choice.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), s)
+        if st:
+            raise RedzoneError(st, "rz_remove_auto_batched")
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, choice
     C = None
     if decide_small:  # every list's method before the batch is planned
         disp.choose_async(descs)

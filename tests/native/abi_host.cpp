@@ -211,6 +211,12 @@ int main() {
               choice[3] == RZ_METHOD_REDZONE,
           "the rule's choices (k = 0 and k = n: Redzone)");
     CHECK(rz_dispatch_collect(ctx, choice, 4) == RZ_EINVAL, "a choice is collected once");
+    CHECK(rz_remove_auto_batched(ctx, nullptr, nullptr, 2, 1 << 22, 0, nullptr, s) == RZ_EINVAL,
+          "rz_remove_auto_batched without lists");
+    CHECK(rz_remove_auto_batched(ctx, nullptr, lists, 4, -1, 0, nullptr, s) == RZ_EINVAL,
+          "rz_remove_auto_batched with a negative threshold");
+    CHECK(rz_remove_auto_batched(ctx, nullptr, bad, 1, 0, 0, nullptr, s) == RZ_EINVAL,
+          "rz_remove_auto_batched with k > n (a large list)");
     HIPCHECK(hipFree(dA));
     HIPCHECK(hipFree(dR));
   }
