@@ -177,6 +177,36 @@ int main() {
       std::vector<uint32_t> out(ns[l]);
       HIPCHECK(hipMemcpy(out.data(), dAs[l], ns[l] * 4, hipMemcpyDeviceToHost));
       check_redzone(out, ns[l], Rs[l], "rz_remove_batched");
+    }
+
+    // 3b. the same lists through rz_remove_auto_batched with a threshold of
+    // 2^12 removals: list 0 in the batch, lists 1 and 2 on their own launches
+    // with the k < n/2 rule (no network is installed in this context), so
+    // list 1 runs Redzone and list 2 (2k = n) the stable partition
+    for (int l = 0; l < 3; ++l) {
+      std::vector<uint32_t> A(ns[l]);
+      std::iota(A.begin(), A.end(), 0u);
+      HIPCHECK(hipMemcpy(dAs[l], A.data(), ns[l] * 4, hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(dRs[l], Rs[l].data(), ks[l] * 4, hipMemcpyHostToDevice));
+    }
+    int32_t method[3] = {-1, -1, -1};
+    CHECK(rz_remove_auto_batched(ctx, nullptr, descs, 3, 1 << 12, 0, method, s) == RZ_OK, "rz_remove_auto_batched");
+    HIPCHECK(hipStreamSynchronize(s));
+    CHECK(method[0] == RZ_METHOD_REDZONE && method[1] == RZ_METHOD_REDZONE && method[2] == RZ_METHOD_PARTITION,
+          "rz_remove_auto_batched methods %d %d %d", method[0], method[1], method[2]);
+    for (int l = 0; l < 3; ++l) {
+      std::vector<uint32_t> out(ns[l]);
+      HIPCHECK(hipMemcpy(out.data(), dAs[l], ns[l] * 4, hipMemcpyDeviceToHost));
+      if (method[l] == RZ_METHOD_PARTITION) {
+        std::vector<char> removed(ns[l], 0);
+        for (int32_t r : Rs[l]) removed[r] = 1;
+        int64_t j = 0, bad = 0;
+        for (int64_t i = 0; i < ns[l]; ++i)
+          if (!removed[i] && out[j++] != (uint32_t)i) ++bad;
+        CHECK(bad == 0 && j == ns[l] - ks[l], "rz_remove_auto_batched partition: %lld out of order", (long long)bad);
+      } else {
+        check_redzone(out, ns[l], Rs[l], "rz_remove_auto_batched");
+      }
       HIPCHECK(hipFree(dAs[l]));
       HIPCHECK(hipFree(dRs[l]));
     }
